@@ -1,0 +1,183 @@
+/*
+ * nhqg.h -- C ABI of the MI355X (gfx950) NHQG time-step library.
+ *
+ * The reference exposes the hot path as two Python calls,
+ *     grid = make_grid(cfg)             nhqg/grid.py:439
+ *     new  = imex_step(state, grid)     nhqg/solver.py:1228
+ * over State(q_hat, w_hat, th_hat, th_bar)  (nhqg/solver.py:33-38), and its
+ * drivers run imex_step in a jit+scan loop (scripts/run_polar.py:338-343,
+ * nhqg/solver.py:1385-1416) followed by compute_diagnostics
+ * (nhqg/diagnostics.py:597-710).  The reference has no native FFI; this
+ * header is the boundary below the Python mirror of that interface
+ * (nhqg_amd/solver.py), and a maintainer binding it from elsewhere needs
+ * only these symbols.
+ *
+ * Conventions: plain pointers and sizes; all arrays row-major; complex
+ * arrays are interleaved float64 pairs (numpy complex128).  Every call
+ * returns an NHQG_* status; nothing aborts.  A context owns one HIP stream
+ * on the current device and is not thread safe.
+ */
+#ifndef NHQG_H
+#define NHQG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NHQG_OK 0
+#define NHQG_ERR_ARG 1         /* bad argument or unsupported shape */
+#define NHQG_ERR_HIP 2         /* HIP runtime error (see nhqg_last_error) */
+#define NHQG_ERR_UNSUPPORTED 3 /* configuration branch not built */
+#define NHQG_ERR_STATE 4       /* call out of order (e.g. step before set_state) */
+
+typedef struct nhqg_ctx nhqg_ctx;
+
+/* Scalar parameters of the step (nhqg/config.py:17-73 after make_grid's
+ * derivations, nhqg/grid.py:669-687). */
+typedef struct {
+  int32_t nx;          /* horizontal grid points, power of two 16..1024 */
+  int32_t nz;          /* Chebyshev intervals; Nz+1 CGL levels, Nz <= 271 */
+  int32_t nkx, nky;    /* kept kx rows / ky columns of the mode layout */
+  int32_t n_shells;    /* number of |k|^2 shells in shell_inv */
+  int32_t n_pairs;     /* entries of herm_pairs */
+  int32_t substeps;    /* SBP corrector substeps (sbp_corrector_substeps) */
+  int32_t evolve_mean; /* 1: evolve_mean + balanced_sbp2_pc corrector */
+  int32_t advection;   /* 0: flux form (the production branch) */
+  int32_t w_neumann;   /* 1: w in the mixed bottom-Dirichlet/top-Neumann basis */
+  int32_t apply_mask;  /* 1: layout keeps out-of-band modes; mask outputs */
+  int32_t reserved;
+  double dt, gamma_imex, ra_sigma, sigma, eps_sq, beta;
+} nhqg_params;
+
+/* Host operator tables, all float64 row-major unless noted.  Nq = nz+1,
+ * Ng = nz-1, nm = nkx*nky modes (ky-major: m = ky_i*nkx + kx_i). */
+typedef struct {
+  /* fused vertical operators (products of the reference's grid matrices) */
+  const double *V;      /* Nq x Nq  coefficients -> CGL values */
+  const double *VSw;    /* Nq x Ng  w Galerkin -> CGL values */
+  const double *VSd;    /* Nq x Ng  theta Galerkin -> CGL values */
+  const double *Vinv;   /* Nq x Nq */
+  const double *Pw;     /* Ng x Nq  w_pinv proj_w V_inv */
+  const double *Pd;     /* Ng x Nq  d_pinv proj_d V_inv */
+  const double *Qw;     /* Ng x Nq  w_pinv proj_w G_Z */
+  const double *GSw;    /* Nq x Ng  G_Z w_stencil */
+  const double *W2S;    /* Nq x Ng  cgl_to_sbp V w_stencil */
+  const double *T2S;    /* Nq x Ng  cgl_to_sbp V d_stencil */
+  const double *S2T;    /* Ng x Nq  d_pinv proj_d V_inv sbp_to_cgl */
+  const double *thb2s;  /* Nq x Nq  cgl_to_sbp V */
+  const double *s2thb;  /* Nq x Nq  proj_d V_inv sbp_to_cgl */
+  const double *proj_d; /* Nq x Nq */
+  const double *d1_band;  /* Nq x 3, SBP D1 tridiagonal band */
+  const double *lap_band; /* Nq x 3, SBP Laplacian tridiagonal band */
+  const double *Mtw;    /* Ng x Ng  w_pinv proj_w d_stencil (w_neumann only) */
+  const double *Mwt;    /* Ng x Ng  map_w_to_th (w_neumann only) */
+  const double *Mth2w;  /* Ng x Ng  map_th_to_w (w_neumann only) */
+  const double *cc;     /* Nq Clenshaw-Curtis weights */
+  const double *row_cc; /* 1 x Nq (cc as a row, for depth averages) */
+  const double *cc_sbp; /* Nq: cc^T sbp_to_cgl (Nusselt on the SBP grid) */
+  const double *Sw;     /* Nq x Ng  w Galerkin stencil */
+  const double *Sd;     /* Nq x Ng  theta Galerkin stencil */
+  /* mode layout */
+  const int32_t *rows;      /* nkx: storage row of each kept kx */
+  const int32_t *kidx_of_x; /* nx: kept index of each row or -1 */
+  const int32_t *full_idx;  /* nm: row*(nx/2+1) + ky of each mode */
+  const double *kx_row;     /* nkx */
+  const double *ky_col;     /* nky */
+  /* per-mode tables (nm) */
+  const double *kx_mode, *ksq, *inv_denom, *inv_alpha_q, *inv_alpha_th;
+  const double *diss_q, *diss_w, *diss_th, *mask, *pweight;
+  const int32_t *partner;   /* nm: (-kx) partner on self-conjugate columns or -1 */
+  const int32_t *herm_pairs; /* 2*n_pairs: (m, m') with m <= m' */
+  /* |k|^2 shells */
+  const int32_t *shell_ptr;   /* n_shells+1 offsets into shell_modes */
+  const int32_t *shell_modes; /* modes grouped by shell */
+  const double *shell_inv;    /* n_shells x Ng x Ng, each TRANSPOSED */
+  /* horizontal physical fields, nx*nx, or NULL */
+  const double *eta_phys;     /* trap background PV on the grid */
+  const double *sponge_phys;  /* band-limited sponge rate */
+  const double *twiddle;      /* 2*nx: exp(-2 pi i k / nx) */
+} nhqg_operators;
+
+/* Diagnostics vector layout returned by nhqg_diagnostics (length
+ * NHQG_DIAG_SCALARS + 3*(nz+1)). */
+#define NHQG_DIAG_KE_TOT 0
+#define NHQG_DIAG_KE_BT 1
+#define NHQG_DIAG_KE_BC 2
+#define NHQG_DIAG_MAX_SPEED 3
+#define NHQG_DIAG_ENSTROPHY 4
+#define NHQG_DIAG_NUSSELT 5
+#define NHQG_DIAG_NUSSELT_D 6
+#define NHQG_DIAG_VOLTW_D 7
+#define NHQG_DIAG_THBAR_MAX 8
+#define NHQG_DIAG_Q_RMS 9
+#define NHQG_DIAG_W_RMS 10
+#define NHQG_DIAG_TH_RMS 11
+#define NHQG_DIAG_SCALARS 12
+
+/* Build a context: uploads operators, allocates the device state and
+ * workspace, captures the step graph lazily.  Replaces make_grid's device
+ * side (nhqg/grid.py:439-789). */
+int nhqg_create(const nhqg_params *p, const nhqg_operators *ops, nhqg_ctx **out);
+int nhqg_destroy(nhqg_ctx *ctx);
+
+/* State in the reference layout (nhqg/solver.py:33-38): q (nz+1, nx, nx/2+1)
+ * complex128, w and th (nz-1, nx, nx/2+1) complex128, th_bar (nz+1,) float64.
+ * Modes outside the layout are ignored on set and returned as zero. */
+int nhqg_set_state(nhqg_ctx *ctx, const double *q, const double *w, const double *th,
+                   const double *th_bar);
+int nhqg_get_state(nhqg_ctx *ctx, double *q, double *w, double *th, double *th_bar);
+
+/* Advance n_steps of imex_step (nhqg/solver.py:1228-1242) on the device
+ * state by replaying the captured step graph; synchronous. */
+int nhqg_step(nhqg_ctx *ctx, int32_t n_steps);
+/* Same, asynchronous on the context stream (call nhqg_sync to wait). */
+int nhqg_step_async(nhqg_ctx *ctx, int32_t n_steps);
+int nhqg_sync(nhqg_ctx *ctx);
+
+/* Judged diagnostics of the current state (nhqg/diagnostics.py:597-675):
+ * out[NHQG_DIAG_*] scalars then q, w, theta vertical spectra. */
+int nhqg_diagnostics(nhqg_ctx *ctx, double *out, int32_t n_out);
+
+/* Component entry points on the device state, for parity testing:
+ *   NHQG_OP_EXPLICIT_RHS    aux <- explicit_rhs(state)    solver.py:510-599
+ *   NHQG_OP_IMPLICIT_TEND   aux <- implicit_tendency(state) solver.py:615-641
+ *   NHQG_OP_IMPLICIT_SOLVE  state <- imex_implicit_solve(state as R)
+ *                           and th_bar <- proj_d th_bar     solver.py:691-744
+ *   NHQG_OP_CORRECTOR       state <- SBP corrector(state, total_dt=arg)
+ *                                                         solver.py:909-950
+ *   NHQG_OP_SANITIZE        state <- finalize + sanitize  solver.py:128-146
+ * nhqg_get_aux copies the aux fields (same layout as the state, th_bar
+ * part returned as zeros). */
+#define NHQG_OP_EXPLICIT_RHS 1
+#define NHQG_OP_IMPLICIT_TEND 2
+#define NHQG_OP_IMPLICIT_SOLVE 3
+#define NHQG_OP_CORRECTOR 4
+#define NHQG_OP_SANITIZE 5
+int nhqg_apply(nhqg_ctx *ctx, int32_t op, double arg);
+int nhqg_get_aux(nhqg_ctx *ctx, double *q, double *w, double *th);
+
+/* Device time of the last nhqg_step call's replays in milliseconds, from
+ * HIP events on the context stream. */
+int nhqg_last_step_ms(nhqg_ctx *ctx, double *ms);
+
+/* Per-kernel-class timing of one eagerly launched step (HIP events around
+ * each group on the context stream): out[0..n) in ms, names in
+ * nhqg_timing_name(i). */
+int nhqg_profile_step(nhqg_ctx *ctx, double *out, int32_t n_out);
+const char *nhqg_timing_name(int32_t i);
+
+/* Bytes of device memory held by the context. */
+int nhqg_device_bytes(nhqg_ctx *ctx, uint64_t *bytes);
+
+/* Last error text of a context (or of the last failed nhqg_create when
+ * ctx is NULL). */
+const char *nhqg_last_error(const nhqg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NHQG_H */

@@ -1,0 +1,240 @@
+"""ctypes binding of the HIP library (include/nhqg.h).
+
+The library ``libnhqg.so`` lives next to this file and is built by
+``nhqg_amd/build.py``.  There is no CPU fallback: if the library or a GPU is
+missing, creating a context raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnhqg.so")
+_lib = None
+
+NHQG_OK = 0
+OP_EXPLICIT_RHS = 1
+OP_IMPLICIT_TEND = 2
+OP_IMPLICIT_SOLVE = 3
+OP_CORRECTOR = 4
+OP_SANITIZE = 5
+DIAG_NAMES = ("KE_tot", "KE_bt", "KE_bc", "max_speed", "enstrophy", "Nusselt",
+              "Nusselt_dealiased", "vol_avg_tw_dealiased", "th_bar_max", "q_rms", "w_rms",
+              "th_rms")
+N_DIAG_SCALARS = len(DIAG_NAMES)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "nx", "nz", "nkx", "nky", "n_shells", "n_pairs", "substeps", "evolve_mean",
+        "advection", "w_neumann", "apply_mask", "reserved")] + [
+        (n, C.c_double) for n in ("dt", "gamma_imex", "ra_sigma", "sigma", "eps_sq", "beta")]
+
+
+_OPS_FIELDS = [
+    ("V", _dp), ("VSw", _dp), ("VSd", _dp), ("Vinv", _dp), ("Pw", _dp), ("Pd", _dp),
+    ("Qw", _dp), ("GSw", _dp), ("W2S", _dp), ("T2S", _dp), ("S2T", _dp), ("thb2s", _dp),
+    ("s2thb", _dp), ("proj_d", _dp), ("d1_band", _dp), ("lap_band", _dp), ("Mtw", _dp),
+    ("Mwt", _dp), ("Mth2w", _dp), ("cc", _dp), ("row_cc", _dp), ("cc_sbp", _dp), ("Sw", _dp),
+    ("Sd", _dp), ("rows", _ip), ("kidx_of_x", _ip), ("full_idx", _ip), ("kx_row", _dp),
+    ("ky_col", _dp), ("kx_mode", _dp), ("ksq", _dp), ("inv_denom", _dp),
+    ("inv_alpha_q", _dp), ("inv_alpha_th", _dp), ("diss_q", _dp), ("diss_w", _dp),
+    ("diss_th", _dp), ("mask", _dp), ("pweight", _dp), ("partner", _ip),
+    ("herm_pairs", _ip), ("shell_ptr", _ip), ("shell_modes", _ip), ("shell_inv", _dp),
+    ("eta_phys", _dp), ("sponge_phys", _dp), ("twiddle", _dp),
+]
+
+
+class Operators(C.Structure):
+    _fields_ = _OPS_FIELDS
+
+
+EXPORTS = (
+    "nhqg_create", "nhqg_destroy", "nhqg_set_state", "nhqg_get_state", "nhqg_step",
+    "nhqg_step_async", "nhqg_sync", "nhqg_diagnostics", "nhqg_apply", "nhqg_get_aux",
+    "nhqg_last_step_ms", "nhqg_profile_step", "nhqg_timing_name", "nhqg_device_bytes",
+    "nhqg_last_error",
+)
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def load():
+    """Load libnhqg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise NativeError(f"{_LIB_PATH} is missing: run `python nhqg_amd/build.py` "
+                          "(or __graft_entry__.build()) first")
+    lib = C.CDLL(_LIB_PATH)
+    vp = C.c_void_p
+    lib.nhqg_create.argtypes = [C.POINTER(Params), C.POINTER(Operators), C.POINTER(vp)]
+    lib.nhqg_destroy.argtypes = [vp]
+    lib.nhqg_set_state.argtypes = [vp, _dp, _dp, _dp, _dp]
+    lib.nhqg_get_state.argtypes = [vp, _dp, _dp, _dp, _dp]
+    lib.nhqg_step.argtypes = [vp, C.c_int32]
+    lib.nhqg_step_async.argtypes = [vp, C.c_int32]
+    lib.nhqg_sync.argtypes = [vp]
+    lib.nhqg_diagnostics.argtypes = [vp, _dp, C.c_int32]
+    lib.nhqg_apply.argtypes = [vp, C.c_int32, C.c_double]
+    lib.nhqg_get_aux.argtypes = [vp, _dp, _dp, _dp]
+    lib.nhqg_last_step_ms.argtypes = [vp, _dp]
+    lib.nhqg_profile_step.argtypes = [vp, _dp, C.c_int32]
+    lib.nhqg_timing_name.argtypes = [C.c_int32]
+    lib.nhqg_timing_name.restype = C.c_char_p
+    lib.nhqg_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+    lib.nhqg_last_error.argtypes = [vp]
+    lib.nhqg_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _ptr(a, kind=_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(kind)
+
+
+class Context:
+    """Device context of one Grid and mode layout."""
+
+    def __init__(self, grid, layout):
+        from nhqg_amd.grid import device_operators
+
+        self.lib = load()
+        self.grid = grid
+        self.layout = layout
+        t = layout.tables
+        ops = device_operators(grid, layout)
+        keep = {}
+
+        def arr(a, dt=np.float64):
+            if a is None:
+                return None
+            b = np.ascontiguousarray(a, dtype=dt)
+            keep[id(b)] = b
+            return b
+
+        o = Operators()
+        for name, typ in _OPS_FIELDS:
+            src = t[name] if name in t else ops.get(name)
+            if typ is _ip:
+                setattr(o, name, _ptr(arr(src, np.int32), _ip))
+            else:
+                setattr(o, name, _ptr(arr(src, np.float64), _dp))
+        p = Params()
+        p.nx, p.nz = grid.Nx, grid.Nz
+        p.nkx, p.nky = layout.nkx, layout.nky
+        p.n_shells = int(t["shell_ids"].size)
+        p.n_pairs = int(t["herm_pairs"].size // 2)
+        p.substeps = max(1, int(grid.sbp_corrector_substeps))
+        p.evolve_mean = 1 if grid.evolve_mean else 0
+        p.advection = 0
+        p.w_neumann = 1 if grid.w_bc_top == "neumann" else 0
+        p.apply_mask = 1 if layout.full else 0
+        p.dt, p.gamma_imex = grid.dt, grid.gamma_imex
+        p.ra_sigma, p.sigma = grid.Ra_sigma, grid.sigma
+        p.eps_sq, p.beta = grid.mean_temp_eps_sq, grid.beta
+        h = C.c_void_p()
+        rc = self.lib.nhqg_create(C.byref(p), C.byref(o), C.byref(h))
+        if rc != NHQG_OK:
+            raise NativeError(f"nhqg_create failed ({rc}): "
+                              f"{self.lib.nhqg_last_error(None).decode()}")
+        self.h = h
+        self.params = p
+
+    def _check(self, rc, what):
+        if rc != NHQG_OK:
+            msg = self.lib.nhqg_last_error(self.h).decode() if self.h else ""
+            raise NativeError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nhqg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shapes(self):
+        g = self.grid
+        return ((g.Nz + 1, g.Nx, g.Nk), (g.Nz - 1, g.Nx, g.Nk), (g.Nz + 1,))
+
+    def set_state(self, q, w, th, thb):
+        sq, sg, sb = self.shapes()
+        q = np.ascontiguousarray(q, dtype=np.complex128)
+        w = np.ascontiguousarray(w, dtype=np.complex128)
+        th = np.ascontiguousarray(th, dtype=np.complex128)
+        thb = np.ascontiguousarray(thb, dtype=np.float64)
+        if q.shape != sq or w.shape != sg or th.shape != sg or thb.shape != sb:
+            raise ValueError("state shapes do not match the grid")
+        self._check(self.lib.nhqg_set_state(self.h, _ptr(q.view(np.float64)),
+                                            _ptr(w.view(np.float64)),
+                                            _ptr(th.view(np.float64)), _ptr(thb)),
+                    "nhqg_set_state")
+
+    def get_state(self):
+        sq, sg, sb = self.shapes()
+        q = np.empty(sq, np.complex128)
+        w = np.empty(sg, np.complex128)
+        th = np.empty(sg, np.complex128)
+        thb = np.empty(sb, np.float64)
+        self._check(self.lib.nhqg_get_state(self.h, _ptr(q.view(np.float64)),
+                                            _ptr(w.view(np.float64)),
+                                            _ptr(th.view(np.float64)), _ptr(thb)),
+                    "nhqg_get_state")
+        return q, w, th, thb
+
+    def step(self, n=1):
+        self._check(self.lib.nhqg_step(self.h, int(n)), "nhqg_step")
+
+    def step_async(self, n=1):
+        self._check(self.lib.nhqg_step_async(self.h, int(n)), "nhqg_step_async")
+
+    def sync(self):
+        self._check(self.lib.nhqg_sync(self.h), "nhqg_sync")
+
+    def last_step_ms(self):
+        v = C.c_double()
+        self._check(self.lib.nhqg_last_step_ms(self.h, C.byref(v)), "nhqg_last_step_ms")
+        return v.value
+
+    def apply(self, op, arg=0.0):
+        self._check(self.lib.nhqg_apply(self.h, int(op), float(arg)), "nhqg_apply")
+
+    def get_aux(self):
+        sq, sg, _ = self.shapes()
+        q = np.empty(sq, np.complex128)
+        w = np.empty(sg, np.complex128)
+        th = np.empty(sg, np.complex128)
+        self._check(self.lib.nhqg_get_aux(self.h, _ptr(q.view(np.float64)),
+                                          _ptr(w.view(np.float64)),
+                                          _ptr(th.view(np.float64))), "nhqg_get_aux")
+        return q, w, th
+
+    def diagnostics(self):
+        n = N_DIAG_SCALARS + 3 * (self.grid.Nz + 1)
+        out = np.zeros(n, np.float64)
+        self._check(self.lib.nhqg_diagnostics(self.h, _ptr(out), n), "nhqg_diagnostics")
+        return out
+
+    def device_bytes(self):
+        v = C.c_uint64()
+        self._check(self.lib.nhqg_device_bytes(self.h, C.byref(v)), "nhqg_device_bytes")
+        return int(v.value)

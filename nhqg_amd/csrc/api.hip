@@ -1,0 +1,868 @@
+// Context, step orchestration and the C ABI (include/nhqg.h).
+//
+// One context = one HIP stream, device-resident state in the band-compact
+// layout, and the whole ARS(2,2,2) balanced_sbp2_pc step
+// (nhqg/solver.py:1167-1225, dispatched by imex_step :1228-1242) captured
+// once into a hipGraph and replayed per step.
+#include <string>
+#include <vector>
+#include <cstring>
+#include <cstdio>
+#include <cmath>
+#include <algorithm>
+
+#include "common.h"
+#include "../../include/nhqg.h"
+
+namespace {
+
+std::string g_create_error;
+
+struct Ops {
+  double *V, *VSw, *VSd, *Vinv, *Pw, *Pd, *Qw, *GSw, *W2S, *T2S, *S2T, *thb2s, *s2thb, *proj_d;
+  double *d1b, *lb, *Mtw, *Mwt, *Mth2w, *cc, *row_cc, *cc_sbp, *Sw, *Sd;
+  int *rows, *kidx_of_x, *full_idx, *partner, *shell_ptr, *shell_modes;
+  int2* pairs;
+  double *kx_row, *ky_col, *kx_mode, *ksq, *inv_denom, *iaq, *iath, *idq, *diss_q, *diss_w,
+      *diss_th, *mask, *pw, *ones, *pw_ke, *pw_cc;
+  double *shell_inv, *eta_phys, *sponge_phys;
+  double2* tw;
+};
+
+}  // namespace
+
+struct nhqg_ctx {
+  nhqg_params P;
+  int Nx = 0, Nk = 0, Nq = 0, Ng = 0, nkx = 0, nky = 0, nm = 0, nm_pad = 0, ld = 0, ldc = 0;
+  int Pred = 0;
+  size_t bytes = 0;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool have_state = false;
+  double last_ms = 0.0;
+  std::vector<void*> allocs;
+  std::string err;
+  Ops o;
+  // state
+  double *yq, *yw, *yth, *ythb;
+  // workspace
+  double *qn, *wn, *thn;
+  double *E1q, *E1w, *E1th, *E2q, *E2w, *E2th;
+  double *Rq, *Rw, *Rth, *Rthb;
+  double *s1q, *s1w, *s1th, *s1thb, *c1th, *c1thb;
+  double *rhs_w, *tmpg;
+  double *wsbp, *thsbp;
+  double *partial, *sbp, *g;
+  double2 *X5, *Y6;
+  size_t fstride;
+  double2* staging;
+  double *diag, *rowmax;
+};
+
+namespace {
+
+#define CK(expr)                                                                      \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      c->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+      return NHQG_ERR_HIP;                                                            \
+    }                                                                                 \
+  } while (0)
+
+template <typename T>
+int dalloc(nhqg_ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  const size_t b = n * sizeof(T);
+  CK(hipMalloc(&q, b > 0 ? b : 16));
+  c->allocs.push_back(q);
+  c->bytes += b;
+  *p = (T*)q;
+  return NHQG_OK;
+}
+
+template <typename T>
+int upload(nhqg_ctx* c, T** p, const T* host, size_t n) {
+  if (!host) {
+    *p = nullptr;
+    return NHQG_OK;
+  }
+  int r = dalloc(c, p, n);
+  if (r) return r;
+  CK(hipMemcpy(*p, host, n * sizeof(T), hipMemcpyHostToDevice));
+  return NHQG_OK;
+}
+
+// per-mode table padded with zeros to nm_pad
+int upload_modes(nhqg_ctx* c, double** p, const double* host) {
+  std::vector<double> v((size_t)c->nm_pad, 0.0);
+  if (host) std::memcpy(v.data(), host, sizeof(double) * c->nm);
+  return upload(c, p, v.data(), v.size());
+}
+
+int field(nhqg_ctx* c, double** p, int nl) {
+  int r = dalloc(c, p, (size_t)nl * c->ld);
+  if (r) return r;
+  CK(hipMemset(*p, 0, (size_t)nl * c->ld * sizeof(double)));
+  return NHQG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step pieces
+// ---------------------------------------------------------------------------
+#define LK(expr)                                                                      \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      c->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+      return NHQG_ERR_HIP;                                                            \
+    }                                                                                 \
+  } while (0)
+
+int gemm(nhqg_ctx* c, int M, int K, const double* A, const double* B, double* C, const Epi& e) {
+  LK(launch_gemm_f64(M, c->ld, K, A, K, B, C, c->ld, e, c->st));
+  return NHQG_OK;
+}
+
+int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
+  LK(launch_combine(nl, c->ld, out, e, c->st));
+  return NHQG_OK;
+}
+
+#define RUN(expr)             \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_) return r_;        \
+  } while (0)
+
+// explicit_rhs on the base grid (nhqg/solver.py:510-599; E_th_bar = 0).
+int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, double* Eq,
+                 double* Ew, double* Eth) {
+  const Ops& o = c->o;
+  RUN(gemm(c, c->Nq, c->Nq, o.V, q, c->qn, epi_make(1.0)));
+  RUN(gemm(c, c->Nq, c->Ng, o.VSw, w, c->wn, epi_make(1.0)));
+  RUN(gemm(c, c->Nq, c->Ng, o.VSd, th, c->thn, epi_make(1.0)));
+  FluxArgs a;
+  a.N = c->Nx;
+  a.nq = c->Nq;
+  a.nkx = c->nkx;
+  a.nky = c->nky;
+  a.ldc = c->ldc;
+  a.qn = (const double2*)c->qn;
+  a.wn = (const double2*)c->wn;
+  a.thn = (const double2*)c->thn;
+  a.aq = (double2*)c->qn;
+  a.aw = (double2*)c->wn;
+  a.ath = (double2*)c->thn;
+  a.kidx_of_x = o.kidx_of_x;
+  a.rows = o.rows;
+  a.kx_row = o.kx_row;
+  a.ky_col = o.ky_col;
+  a.inv_denom = o.inv_denom;
+  a.mask = o.mask;
+  a.eta_phys = o.eta_phys;
+  a.sponge_phys = o.sponge_phys;
+  a.tw = o.tw;
+  a.X5 = c->X5;
+  a.Y6 = c->Y6;
+  a.fstride = c->fstride;
+  LK(launch_flux(a, c->st));
+  RUN(gemm(c, c->Nq, c->Nq, o.Vinv, c->qn, Eq, epi_make(-1.0)));
+  if (c->P.beta != 0.0)
+    LK(launch_beta(c->Nq, c->ldc, c->nm, c->P.beta, o.kx_mode, o.inv_denom, (const double2*)q,
+                   (double2*)Eq, c->st));
+  RUN(gemm(c, c->Ng, c->Nq, o.Pw, c->wn, Ew, epi_make(-1.0)));
+  RUN(gemm(c, c->Ng, c->Nq, o.Pd, c->thn, Eth, epi_make(-1.0)));
+  return NHQG_OK;
+}
+
+// imex_implicit_solve (nhqg/solver.py:691-738), q_boundary='none'.
+int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double* Rth,
+                   double* oq, double* ow, double* oth) {
+  const Ops& o = c->o;
+  const double al = c->P.gamma_imex * c->P.dt;
+  const double ars = al * c->P.ra_sigma;
+  if (!c->P.w_neumann) {
+    Epi e = epi_make(al);
+    e.s_acc = o.idq;
+    epi_add(e, 1.0, Rw);
+    epi_add(e, ars, Rth, o.iath);
+    RUN(gemm(c, c->Ng, c->Nq, o.Qw, Rq, c->rhs_w, e));
+    LK(launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
+                          c->rhs_w, ow, Rth, oth, o.iath, al, c->st));
+  } else {
+    RUN(gemm(c, c->Ng, c->Ng, o.Mth2w, Rth, c->tmpg, epi_make(1.0)));
+    Epi e = epi_make(al);
+    e.s_acc = o.idq;
+    epi_add(e, 1.0, Rw);
+    epi_add(e, ars, c->tmpg, o.iath);
+    RUN(gemm(c, c->Ng, c->Nq, o.Qw, Rq, c->rhs_w, e));
+    // the fused theta output is overwritten below with the mapped form
+    LK(launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
+                          c->rhs_w, ow, Rth, c->tmpg, o.iath, al, c->st));
+    Epi et = epi_make(al);
+    epi_add(et, 1.0, Rth);
+    et.s_out = o.iath;
+    RUN(gemm(c, c->Ng, c->Ng, o.Mwt, ow, oth, et));
+  }
+  Epi eq = epi_make(al);
+  epi_add(eq, 1.0, Rq);
+  eq.s_out = o.iaq;
+  RUN(gemm(c, c->Nq, c->Ng, o.GSw, ow, oq, eq));
+  return NHQG_OK;
+}
+
+// SBP2 thermal corrector with hoisted invariants (nhqg/solver.py:909-950).
+// th_out may equal th_in; thb_out is written (thb_in copied first).
+int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double* thb_in,
+              double* th_out, double* thb_out, double total_dt) {
+  const Ops& o = c->o;
+  const int n = c->Nq;
+  const int ns = c->P.substeps > 0 ? c->P.substeps : 1;
+  const double sdt = total_dt / ns;
+  const double mu = c->P.eps_sq, kappa = 1.0 / c->P.sigma;
+  RUN(gemm(c, n, c->Ng, o.W2S, w_in, c->wsbp, epi_make(1.0)));
+  LK(launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->wsbp, o.pw, o.partner, c->partial,
+                     c->st));
+  LK(launch_sbp_prepare(n, c->Pred, c->partial, o.d1b, o.lb, 0.5 * mu * kappa * sdt,
+                        0.25 * mu * sdt * sdt, c->sbp, c->st));
+  if (thb_out != thb_in) LK(launch_small_axpy(n, thb_in, thb_in, thb_in, 0.0, thb_out, c->st));
+  const double* cur = th_in;
+  for (int s = 0; s < ns; ++s) {
+    RUN(gemm(c, n, c->Ng, o.T2S, cur, c->thsbp, epi_make(1.0)));
+    LK(launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->thsbp, o.pw, o.partner, c->partial,
+                       c->st));
+    LK(launch_sbp_substep(n, c->Pred, c->partial, o.d1b, o.thb2s, o.s2thb, mu * sdt, c->sbp,
+                          thb_out, c->g, c->st));
+    LK(launch_sbp_update(n, c->ld, c->thsbp, c->wsbp, c->g, sdt, c->st));
+    RUN(gemm(c, c->Ng, n, o.S2T, c->thsbp, th_out, epi_make(1.0)));
+    cur = th_out;
+  }
+  return NHQG_OK;
+}
+
+int sanitize(nhqg_ctx* c) {
+  const Ops& o = c->o;
+  LK(launch_sanitize(c->Nq, c->ld, c->P.n_pairs, o.pairs, 0, c->yq, c->st));
+  LK(launch_sanitize(c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yw, c->st));
+  LK(launch_sanitize(c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yth, c->st));
+  if (c->P.apply_mask) {
+    LK(launch_mask(c->Nq, c->ld, c->yq, o.mask, c->st));
+    LK(launch_mask(c->Ng, c->ld, c->yw, o.mask, c->st));
+    LK(launch_mask(c->Ng, c->ld, c->yth, o.mask, c->st));
+  }
+  return NHQG_OK;
+}
+
+// One full step y <- imex_step(y).
+int enqueue_step(nhqg_ctx* c) {
+  const Ops& o = c->o;
+  const double dt = c->P.dt;
+  const double g = c->P.gamma_imex;
+  const double al = g * dt;
+  const double om = dt * (1.0 - g);
+  const double delta = -0.70710678118654752440;  // -sqrt(2)/2
+  const int Nq = c->Nq, Ng = c->Ng;
+  const bool em = c->P.evolve_mean != 0;
+
+  // stage 1 predictor
+  RUN(explicit_rhs(c, c->yq, c->yw, c->yth, c->E1q, c->E1w, c->E1th));
+  {
+    Epi e = epi_make(0.0);
+    epi_add(e, 1.0, c->yq);
+    epi_add(e, al, c->E1q);
+    RUN(combine(c, Nq, c->Rq, e));
+    Epi f = epi_make(0.0);
+    epi_add(f, 1.0, c->yw);
+    epi_add(f, al, c->E1w);
+    RUN(combine(c, Ng, c->Rw, f));
+    Epi h = epi_make(0.0);
+    epi_add(h, 1.0, c->yth);
+    epi_add(h, al, c->E1th);
+    RUN(combine(c, Ng, c->Rth, h));
+  }
+  RUN(implicit_solve(c, c->Rq, c->Rw, c->Rth, c->s1q, c->s1w, c->s1th));
+  LK(launch_small_matvec(Nq, Nq, o.proj_d, c->ythb, c->s1thb, c->st));
+  // stage 1 corrector
+  const double* th1 = c->s1th;
+  const double* thb1 = c->s1thb;
+  if (em) {
+    RUN(corrector(c, c->s1w, c->s1th, c->s1thb, c->c1th, c->c1thb, al));
+    th1 = c->c1th;
+    thb1 = c->c1thb;
+  }
+  // stage 2
+  RUN(explicit_rhs(c, c->s1q, c->s1w, th1, c->E2q, c->E2w, c->E2th));
+  {
+    // R_q2 = y + dt(delta E1 + (1-delta) E2) + om*(G_Z S_w w1) - om*diss_q*q1
+    Epi e = epi_make(om);
+    epi_add(e, 1.0, c->yq);
+    epi_add(e, dt * delta, c->E1q);
+    epi_add(e, dt * (1.0 - delta), c->E2q);
+    epi_add(e, -om, c->s1q, o.diss_q);
+    RUN(gemm(c, Nq, Ng, o.GSw, c->s1w, c->Rq, e));
+    // R_w2 = ... + om*(inv_denom Qw q1 + Ra_s Mtw th1) - om diss_w w1
+    const double* thw = th1;
+    if (c->P.w_neumann) {
+      RUN(gemm(c, Ng, Ng, o.Mtw, th1, c->tmpg, epi_make(1.0)));
+      thw = c->tmpg;
+    }
+    Epi f = epi_make(om);
+    f.s_acc = o.inv_denom;
+    epi_add(f, 1.0, c->yw);
+    epi_add(f, dt * delta, c->E1w);
+    epi_add(f, dt * (1.0 - delta), c->E2w);
+    epi_add(f, om * c->P.ra_sigma, thw);
+    epi_add(f, -om, c->s1w, o.diss_w);
+    RUN(gemm(c, Ng, Nq, o.Qw, c->s1q, c->Rw, f));
+    // R_th2 = ... + om*I_th - om diss_th th1 + om*(th1 - th1p)/alpha
+    const double* ith = c->s1w;
+    if (c->P.w_neumann) {
+      RUN(gemm(c, Ng, Ng, o.Mwt, c->s1w, c->tmpg, epi_make(1.0)));
+      ith = c->tmpg;
+    }
+    Epi h = epi_make(0.0);
+    epi_add(h, 1.0, c->yth);
+    epi_add(h, dt * delta, c->E1th);
+    epi_add(h, dt * (1.0 - delta), c->E2th);
+    epi_add(h, om, ith);
+    epi_add(h, -om, th1, o.diss_th);
+    if (em) {
+      epi_add(h, om / al, th1);
+      epi_add(h, -om / al, c->s1th);
+    }
+    RUN(combine(c, Ng, c->Rth, h));
+    if (em)
+      LK(launch_small_axpy(Nq, c->ythb, thb1, c->s1thb, om / al, c->Rthb, c->st));
+    else
+      LK(launch_small_axpy(Nq, c->ythb, c->ythb, c->ythb, 0.0, c->Rthb, c->st));
+  }
+  RUN(implicit_solve(c, c->Rq, c->Rw, c->Rth, c->yq, c->yw, c->yth));
+  LK(launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->ythb, c->st));
+  if (em) RUN(corrector(c, c->yw, c->yth, c->ythb, c->yth, c->ythb, al));
+  RUN(sanitize(c));
+  return NHQG_OK;
+}
+
+int build_graph(nhqg_ctx* c) {
+  if (c->gexec) return NHQG_OK;
+  CK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  int r = enqueue_step(c);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(c->st, &gr);
+  if (r) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return r;
+  }
+  if (e != hipSuccess) {
+    c->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+    return NHQG_ERR_HIP;
+  }
+  c->graph = gr;
+  CK(hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0));
+  return NHQG_OK;
+}
+
+int check_params(const nhqg_params* p, std::string& err) {
+  const int nx = p->nx;
+  if (nx < 16 || nx > 1024 || (nx & (nx - 1))) {
+    err = "nx must be a power of two in [16, 1024]";
+    return NHQG_ERR_ARG;
+  }
+  if (p->nz < 3 || p->nz + 1 > 272) {
+    err = "nz must be in [3, 271]";
+    return NHQG_ERR_ARG;
+  }
+  if (p->nkx < 1 || p->nkx > nx || p->nky < 1 || p->nky > nx / 2 + 1) {
+    err = "mode layout out of range";
+    return NHQG_ERR_ARG;
+  }
+  if (p->advection != 0) {
+    err = "only the flux advection form is built (advection=0)";
+    return NHQG_ERR_UNSUPPORTED;
+  }
+  if (p->n_shells < 1) {
+    err = "n_shells must be >= 1";
+    return NHQG_ERR_ARG;
+  }
+  return NHQG_OK;
+}
+
+int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
+  c->P = *p;
+  c->Nx = p->nx;
+  c->Nk = p->nx / 2 + 1;
+  c->Nq = p->nz + 1;
+  c->Ng = p->nz - 1;
+  c->nkx = p->nkx;
+  c->nky = p->nky;
+  c->nm = p->nkx * p->nky;
+  c->nm_pad = (c->nm + 31) / 32 * 32;
+  c->ld = 2 * c->nm_pad;
+  c->ldc = c->nm_pad;
+  c->Pred = 16;
+  CK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  CK(hipEventCreate(&c->ev0));
+  CK(hipEventCreate(&c->ev1));
+  const int Nq = c->Nq, Ng = c->Ng, nm = c->nm;
+  Ops& o = c->o;
+  RUN(upload(c, &o.V, h->V, (size_t)Nq * Nq));
+  RUN(upload(c, &o.VSw, h->VSw, (size_t)Nq * Ng));
+  RUN(upload(c, &o.VSd, h->VSd, (size_t)Nq * Ng));
+  RUN(upload(c, &o.Vinv, h->Vinv, (size_t)Nq * Nq));
+  RUN(upload(c, &o.Pw, h->Pw, (size_t)Ng * Nq));
+  RUN(upload(c, &o.Pd, h->Pd, (size_t)Ng * Nq));
+  RUN(upload(c, &o.Qw, h->Qw, (size_t)Ng * Nq));
+  RUN(upload(c, &o.GSw, h->GSw, (size_t)Nq * Ng));
+  RUN(upload(c, &o.W2S, h->W2S, (size_t)Nq * Ng));
+  RUN(upload(c, &o.T2S, h->T2S, (size_t)Nq * Ng));
+  RUN(upload(c, &o.S2T, h->S2T, (size_t)Ng * Nq));
+  RUN(upload(c, &o.thb2s, h->thb2s, (size_t)Nq * Nq));
+  RUN(upload(c, &o.s2thb, h->s2thb, (size_t)Nq * Nq));
+  RUN(upload(c, &o.proj_d, h->proj_d, (size_t)Nq * Nq));
+  RUN(upload(c, &o.d1b, h->d1_band, (size_t)Nq * 3));
+  RUN(upload(c, &o.lb, h->lap_band, (size_t)Nq * 3));
+  RUN(upload(c, &o.cc, h->cc, (size_t)Nq));
+  RUN(upload(c, &o.row_cc, h->row_cc, (size_t)Nq));
+  RUN(upload(c, &o.cc_sbp, h->cc_sbp, (size_t)Nq));
+  RUN(upload(c, &o.Sw, h->Sw, (size_t)Nq * Ng));
+  RUN(upload(c, &o.Sd, h->Sd, (size_t)Nq * Ng));
+  if (p->w_neumann) {
+    if (!h->Mtw || !h->Mwt || !h->Mth2w) {
+      c->err = "w_neumann requires Mtw, Mwt and Mth2w";
+      return NHQG_ERR_ARG;
+    }
+    RUN(upload(c, &o.Mtw, h->Mtw, (size_t)Ng * Ng));
+    RUN(upload(c, &o.Mwt, h->Mwt, (size_t)Ng * Ng));
+    RUN(upload(c, &o.Mth2w, h->Mth2w, (size_t)Ng * Ng));
+  } else {
+    o.Mtw = o.Mwt = o.Mth2w = nullptr;
+  }
+  RUN(upload(c, &o.rows, h->rows, (size_t)c->nkx));
+  RUN(upload(c, &o.kidx_of_x, h->kidx_of_x, (size_t)c->Nx));
+  RUN(upload(c, &o.full_idx, h->full_idx, (size_t)nm));
+  RUN(upload(c, &o.kx_row, h->kx_row, (size_t)c->nkx));
+  RUN(upload(c, &o.ky_col, h->ky_col, (size_t)c->nky));
+  RUN(upload_modes(c, &o.kx_mode, h->kx_mode));
+  RUN(upload_modes(c, &o.ksq, h->ksq));
+  RUN(upload_modes(c, &o.inv_denom, h->inv_denom));
+  RUN(upload_modes(c, &o.iaq, h->inv_alpha_q));
+  RUN(upload_modes(c, &o.iath, h->inv_alpha_th));
+  RUN(upload_modes(c, &o.diss_q, h->diss_q));
+  RUN(upload_modes(c, &o.diss_w, h->diss_w));
+  RUN(upload_modes(c, &o.diss_th, h->diss_th));
+  RUN(upload_modes(c, &o.mask, h->mask));
+  RUN(upload_modes(c, &o.pw, h->pweight));
+  {
+    std::vector<double> idq((size_t)nm), ones((size_t)nm, 1.0), pke((size_t)nm);
+    for (int m = 0; m < nm; ++m) {
+      idq[m] = h->inv_denom[m] * h->inv_alpha_q[m];
+      pke[m] = h->pweight[m] * h->ksq[m] * h->inv_denom[m] * h->inv_denom[m];
+    }
+    RUN(upload_modes(c, &o.idq, idq.data()));
+    RUN(upload_modes(c, &o.ones, ones.data()));
+    RUN(upload_modes(c, &o.pw_ke, pke.data()));
+  }
+  {
+    std::vector<int> part((size_t)c->nm_pad, -1);
+    std::memcpy(part.data(), h->partner, sizeof(int) * nm);
+    RUN(upload(c, &o.partner, part.data(), part.size()));
+  }
+  RUN(upload(c, (int**)&o.pairs, h->herm_pairs, (size_t)2 * p->n_pairs));
+  RUN(upload(c, &o.shell_ptr, h->shell_ptr, (size_t)p->n_shells + 1));
+  RUN(upload(c, &o.shell_modes, h->shell_modes, (size_t)nm));
+  RUN(upload(c, &o.shell_inv, h->shell_inv, (size_t)p->n_shells * Ng * Ng));
+  RUN(upload(c, &o.eta_phys, h->eta_phys, (size_t)c->Nx * c->Nx));
+  RUN(upload(c, &o.sponge_phys, h->sponge_phys, (size_t)c->Nx * c->Nx));
+  RUN(upload(c, (double**)&o.tw, h->twiddle, (size_t)2 * c->Nx));
+
+  // state and workspace
+  RUN(field(c, &c->yq, Nq));
+  RUN(field(c, &c->yw, Ng));
+  RUN(field(c, &c->yth, Ng));
+  RUN(dalloc(c, &c->ythb, (size_t)Nq));
+  RUN(field(c, &c->qn, Nq));
+  RUN(field(c, &c->wn, Nq));
+  RUN(field(c, &c->thn, Nq));
+  RUN(field(c, &c->E1q, Nq));
+  RUN(field(c, &c->E1w, Ng));
+  RUN(field(c, &c->E1th, Ng));
+  RUN(field(c, &c->E2q, Nq));
+  RUN(field(c, &c->E2w, Ng));
+  RUN(field(c, &c->E2th, Ng));
+  RUN(field(c, &c->Rq, Nq));
+  RUN(field(c, &c->Rw, Ng));
+  RUN(field(c, &c->Rth, Ng));
+  RUN(dalloc(c, &c->Rthb, (size_t)Nq));
+  RUN(field(c, &c->s1q, Nq));
+  RUN(field(c, &c->s1w, Ng));
+  RUN(field(c, &c->s1th, Ng));
+  RUN(dalloc(c, &c->s1thb, (size_t)Nq));
+  RUN(field(c, &c->c1th, Ng));
+  RUN(dalloc(c, &c->c1thb, (size_t)Nq));
+  RUN(field(c, &c->rhs_w, Ng));
+  RUN(field(c, &c->tmpg, Ng));
+  RUN(field(c, &c->wsbp, Nq));
+  RUN(field(c, &c->thsbp, Nq));
+  RUN(dalloc(c, &c->partial, (size_t)Nq * c->Pred + 64));
+  RUN(dalloc(c, &c->sbp, (size_t)Nq * 16));
+  RUN(dalloc(c, &c->g, (size_t)Nq));
+  c->fstride = (size_t)Nq * c->Nx * c->nky;
+  RUN(dalloc(c, &c->X5, c->fstride * 5));
+  RUN(dalloc(c, &c->Y6, c->fstride * 6));
+  RUN(dalloc(c, &c->staging, (size_t)Nq * c->Nx * c->Nk));
+  RUN(dalloc(c, &c->diag, (size_t)64 + 8 * Nq));
+  RUN(dalloc(c, &c->rowmax, (size_t)Nq * c->Nx));
+  CK(hipMemset(c->ythb, 0, sizeof(double) * Nq));
+  CK(hipDeviceSynchronize());
+  return NHQG_OK;
+}
+
+int copy_full_in(nhqg_ctx* c, const double* host, int nl, double* comp) {
+  const size_t full_n = (size_t)c->Nx * c->Nk;
+  CK(hipMemcpyAsync(c->staging, host, nl * full_n * sizeof(double2), hipMemcpyHostToDevice,
+                    c->st));
+  LK(launch_gather(nl, c->nm, c->nm_pad, (int)full_n, c->o.full_idx, c->staging,
+                   (double2*)comp, c->st));
+  CK(hipStreamSynchronize(c->st));
+  return NHQG_OK;
+}
+
+int copy_full_out(nhqg_ctx* c, double* host, int nl, const double* comp) {
+  const size_t full_n = (size_t)c->Nx * c->Nk;
+  LK(launch_scatter(nl, c->nm, c->nm_pad, (int)full_n, c->o.full_idx, (const double2*)comp,
+                    c->staging, c->st));
+  CK(hipMemcpyAsync(host, c->staging, nl * full_n * sizeof(double2), hipMemcpyDeviceToHost,
+                    c->st));
+  CK(hipStreamSynchronize(c->st));
+  return NHQG_OK;
+}
+
+// Sum of partial[l][p] weighted by wl[l] (host side, Pred partials per level).
+double reduce_partials(const std::vector<double>& part, int nl, int P, const double* wl) {
+  double s = 0.0;
+  for (int l = 0; l < nl; ++l) {
+    double t = 0.0;
+    for (int p = 0; p < P; ++p) t += part[(size_t)l * P + p];
+    s += (wl ? wl[l] : 1.0) * t;
+  }
+  return s;
+}
+
+int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
+  const Ops& o = c->o;
+  const int Nq = c->Nq, Ng = c->Ng, P = c->Pred;
+  if (n_out < NHQG_DIAG_SCALARS + 3 * Nq) {
+    c->err = "diagnostics output too short";
+    return NHQG_ERR_ARG;
+  }
+  std::vector<double> cc(Nq), part((size_t)Nq * P);
+  CK(hipMemcpy(cc.data(), o.cc, sizeof(double) * Nq, hipMemcpyDeviceToHost));
+  auto fetch = [&](int nl) -> int {
+    CK(hipStreamSynchronize(c->st));
+    CK(hipMemcpy(part.data(), c->partial, sizeof(double) * nl * P, hipMemcpyDeviceToHost));
+    return NHQG_OK;
+  };
+  // nodal q (psi = -q inv_denom, applied through the per-mode weights)
+  RUN(gemm(c, Nq, Nq, o.V, c->yq, c->qn, epi_make(1.0)));
+  LK(launch_parseval(Nq, c->nm, c->ld, P, c->qn, c->qn, o.pw_ke, nullptr, c->partial, c->st));
+  RUN(fetch(Nq));
+  const double ke_tot = 0.5 * reduce_partials(part, Nq, P, cc.data());
+  LK(launch_parseval(Nq, c->nm, c->ld, P, c->qn, c->qn, o.pw, nullptr, c->partial, c->st));
+  RUN(fetch(Nq));
+  const double q2 = reduce_partials(part, Nq, P, cc.data());
+  // barotropic: depth average of nodal q, then the same weights
+  RUN(gemm(c, 1, Nq, o.row_cc, c->qn, c->E2q, epi_make(1.0)));
+  LK(launch_parseval(1, c->nm, c->ld, P, c->E2q, c->E2q, o.pw_ke, nullptr, c->partial, c->st));
+  RUN(fetch(1));
+  const double ke_bt = 0.5 * reduce_partials(part, 1, P, nullptr);
+  // max speed from the nodal psi
+  {
+    FluxArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.N = c->Nx;
+    a.nq = Nq;
+    a.nkx = c->nkx;
+    a.nky = c->nky;
+    a.ldc = c->ldc;
+    a.qn = (const double2*)c->qn;
+    a.kidx_of_x = o.kidx_of_x;
+    a.rows = o.rows;
+    a.kx_row = o.kx_row;
+    a.ky_col = o.ky_col;
+    a.inv_denom = o.inv_denom;
+    a.tw = o.tw;
+    a.X5 = c->X5;
+    a.Y6 = c->Y6;
+    a.fstride = c->fstride;
+    LK(launch_speed(a, c->rowmax, c->st));
+    LK(launch_max_reduce((size_t)Nq * c->Nx, c->rowmax, c->diag, c->st));
+  }
+  double max_speed = 0.0;
+  CK(hipStreamSynchronize(c->st));
+  CK(hipMemcpy(&max_speed, c->diag, sizeof(double), hipMemcpyDeviceToHost));
+  // w, theta nodal
+  RUN(gemm(c, Nq, Ng, o.VSw, c->yw, c->wn, epi_make(1.0)));
+  RUN(gemm(c, Nq, Ng, o.VSd, c->yth, c->thn, epi_make(1.0)));
+  LK(launch_parseval(Nq, c->nm, c->ld, P, c->wn, c->thn, o.pw, nullptr, c->partial, c->st));
+  RUN(fetch(Nq));
+  const double vol_tw = reduce_partials(part, Nq, P, cc.data());
+  LK(launch_parseval(Nq, c->nm, c->ld, P, c->wn, c->wn, o.pw, nullptr, c->partial, c->st));
+  RUN(fetch(Nq));
+  const double w2 = reduce_partials(part, Nq, P, cc.data());
+  LK(launch_parseval(Nq, c->nm, c->ld, P, c->thn, c->thn, o.pw, nullptr, c->partial, c->st));
+  RUN(fetch(Nq));
+  const double t2 = reduce_partials(part, Nq, P, cc.data());
+  // solver-consistent flux profile -> Nu_d (diagnostics.py:206-212, 655-675):
+  // vol = sum_j cc_j prof_j with prof = sbp_to_cgl <w_sbp th_sbp> on the SBP
+  // branch, so the host folds cc^T sbp_to_cgl into cc_sbp.
+  double vol_d = 0.0;
+  {
+    const double* wl = cc.data();
+    std::vector<double> ccs(Nq);
+    if (c->P.evolve_mean) {
+      RUN(gemm(c, Nq, Ng, o.W2S, c->yw, c->wsbp, epi_make(1.0)));
+      RUN(gemm(c, Nq, Ng, o.T2S, c->yth, c->thsbp, epi_make(1.0)));
+      LK(launch_parseval(Nq, c->nm, c->ld, P, c->wsbp, c->thsbp, o.pw, o.partner, c->partial,
+                         c->st));
+      CK(hipMemcpy(ccs.data(), o.cc_sbp, sizeof(double) * Nq, hipMemcpyDeviceToHost));
+      wl = ccs.data();
+    } else {
+      LK(launch_parseval(Nq, c->nm, c->ld, P, c->wn, c->thn, o.pw, o.partner, c->partial,
+                         c->st));
+    }
+    RUN(fetch(Nq));
+    vol_d = reduce_partials(part, Nq, P, wl);
+  }
+  // vertical spectra: raw sums of |f_n|^2 of q, w_cheb, th_cheb
+  // (diagnostics.py:583-585)
+  std::vector<double> spec(3 * (size_t)Nq, 0.0);
+  RUN(gemm(c, Nq, Ng, o.Sw, c->yw, c->wn, epi_make(1.0)));
+  RUN(gemm(c, Nq, Ng, o.Sd, c->yth, c->thn, epi_make(1.0)));
+  const double* srcs[3] = {c->yq, c->wn, c->thn};
+  for (int f = 0; f < 3; ++f) {
+    LK(launch_parseval(Nq, c->nm, c->ld, P, srcs[f], srcs[f], o.ones, nullptr, c->partial, c->st));
+    RUN(fetch(Nq));
+    for (int l = 0; l < Nq; ++l) {
+      double t = 0.0;
+      for (int q = 0; q < P; ++q) t += part[(size_t)l * P + q];
+      spec[(size_t)f * Nq + l] = t;
+    }
+  }
+  double thb_max = 0.0;
+  {
+    std::vector<double> tb(Nq);
+    CK(hipMemcpy(tb.data(), c->ythb, sizeof(double) * Nq, hipMemcpyDeviceToHost));
+    for (double v : tb) thb_max = std::max(thb_max, std::fabs(v));
+  }
+  for (int i = 0; i < n_out; ++i) out[i] = 0.0;
+  out[NHQG_DIAG_KE_TOT] = ke_tot;
+  out[NHQG_DIAG_KE_BT] = ke_bt;
+  out[NHQG_DIAG_KE_BC] = ke_tot - ke_bt;
+  out[NHQG_DIAG_MAX_SPEED] = max_speed;
+  out[NHQG_DIAG_ENSTROPHY] = 0.5 * q2;
+  out[NHQG_DIAG_NUSSELT] = 1.0 + vol_tw;
+  out[NHQG_DIAG_NUSSELT_D] = 1.0 + vol_d;
+  out[NHQG_DIAG_VOLTW_D] = vol_d;
+  for (int i = 0; i < 3 * Nq; ++i) out[NHQG_DIAG_SCALARS + i] = spec[i];
+  out[NHQG_DIAG_THBAR_MAX] = thb_max;
+  out[NHQG_DIAG_Q_RMS] = std::sqrt(q2);
+  out[NHQG_DIAG_W_RMS] = std::sqrt(w2);
+  out[NHQG_DIAG_TH_RMS] = std::sqrt(t2);
+  return NHQG_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int nhqg_create(const nhqg_params* p, const nhqg_operators* ops, nhqg_ctx** out) {
+  if (!p || !ops || !out) {
+    g_create_error = "null argument";
+    return NHQG_ERR_ARG;
+  }
+  *out = nullptr;
+  std::string err;
+  int r = check_params(p, err);
+  if (r) {
+    g_create_error = err;
+    return r;
+  }
+  nhqg_ctx* c = new nhqg_ctx();
+  r = create_impl(c, p, ops);
+  if (r) {
+    g_create_error = c->err;
+    nhqg_destroy(c);
+    return r;
+  }
+  *out = c;
+  return NHQG_OK;
+}
+
+int nhqg_destroy(nhqg_ctx* c) {
+  if (!c) return NHQG_OK;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return NHQG_OK;
+}
+
+int nhqg_set_state(nhqg_ctx* c, const double* q, const double* w, const double* th,
+                   const double* th_bar) {
+  if (!c || !q || !w || !th || !th_bar) return NHQG_ERR_ARG;
+  RUN(copy_full_in(c, q, c->Nq, c->yq));
+  RUN(copy_full_in(c, w, c->Ng, c->yw));
+  RUN(copy_full_in(c, th, c->Ng, c->yth));
+  CK(hipMemcpy(c->ythb, th_bar, sizeof(double) * c->Nq, hipMemcpyHostToDevice));
+  c->have_state = true;
+  return NHQG_OK;
+}
+
+int nhqg_get_state(nhqg_ctx* c, double* q, double* w, double* th, double* th_bar) {
+  if (!c || !q || !w || !th || !th_bar) return NHQG_ERR_ARG;
+  CK(hipStreamSynchronize(c->st));
+  RUN(copy_full_out(c, q, c->Nq, c->yq));
+  RUN(copy_full_out(c, w, c->Ng, c->yw));
+  RUN(copy_full_out(c, th, c->Ng, c->yth));
+  CK(hipMemcpy(th_bar, c->ythb, sizeof(double) * c->Nq, hipMemcpyDeviceToHost));
+  return NHQG_OK;
+}
+
+int nhqg_step_async(nhqg_ctx* c, int32_t n) {
+  if (!c) return NHQG_ERR_ARG;
+  if (!c->have_state) {
+    c->err = "nhqg_step before nhqg_set_state";
+    return NHQG_ERR_STATE;
+  }
+  if (n <= 0) return NHQG_OK;
+  RUN(build_graph(c));
+  CK(hipEventRecord(c->ev0, c->st));
+  for (int i = 0; i < n; ++i) CK(hipGraphLaunch(c->gexec, c->st));
+  CK(hipEventRecord(c->ev1, c->st));
+  return NHQG_OK;
+}
+
+int nhqg_sync(nhqg_ctx* c) {
+  if (!c) return NHQG_ERR_ARG;
+  CK(hipStreamSynchronize(c->st));
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last_ms = ms;
+  return NHQG_OK;
+}
+
+int nhqg_step(nhqg_ctx* c, int32_t n) {
+  RUN(nhqg_step_async(c, n));
+  return nhqg_sync(c);
+}
+
+int nhqg_last_step_ms(nhqg_ctx* c, double* ms) {
+  if (!c || !ms) return NHQG_ERR_ARG;
+  *ms = c->last_ms;
+  return NHQG_OK;
+}
+
+int nhqg_diagnostics(nhqg_ctx* c, double* out, int32_t n_out) {
+  if (!c || !out) return NHQG_ERR_ARG;
+  if (!c->have_state) return NHQG_ERR_STATE;
+  return diagnostics_impl(c, out, n_out);
+}
+
+int nhqg_apply(nhqg_ctx* c, int32_t op, double arg) {
+  if (!c) return NHQG_ERR_ARG;
+  if (!c->have_state) return NHQG_ERR_STATE;
+  switch (op) {
+    case NHQG_OP_EXPLICIT_RHS:
+      RUN(explicit_rhs(c, c->yq, c->yw, c->yth, c->E1q, c->E1w, c->E1th));
+      break;
+    case NHQG_OP_IMPLICIT_TEND: {
+      // I_q = G_Z S_w w; I_w = inv_denom Qw q + Ra_s (Mtw) th; I_th = (Mwt) w
+      const Ops& o = c->o;
+      RUN(gemm(c, c->Nq, c->Ng, o.GSw, c->yw, c->E1q, epi_make(1.0)));
+      const double* thw = c->yth;
+      if (c->P.w_neumann) {
+        RUN(gemm(c, c->Ng, c->Ng, o.Mtw, c->yth, c->tmpg, epi_make(1.0)));
+        thw = c->tmpg;
+      }
+      Epi f = epi_make(1.0);
+      f.s_acc = o.inv_denom;
+      epi_add(f, c->P.ra_sigma, thw);
+      RUN(gemm(c, c->Ng, c->Nq, o.Qw, c->yq, c->E1w, f));
+      if (c->P.w_neumann) {
+        RUN(gemm(c, c->Ng, c->Ng, o.Mwt, c->yw, c->E1th, epi_make(1.0)));
+      } else {
+        Epi h = epi_make(0.0);
+        epi_add(h, 1.0, c->yw);
+        RUN(combine(c, c->Ng, c->E1th, h));
+      }
+      break;
+    }
+    case NHQG_OP_IMPLICIT_SOLVE: {
+      Epi e = epi_make(0.0);
+      epi_add(e, 1.0, c->yq);
+      RUN(combine(c, c->Nq, c->Rq, e));
+      Epi f = epi_make(0.0);
+      epi_add(f, 1.0, c->yw);
+      RUN(combine(c, c->Ng, c->Rw, f));
+      Epi h = epi_make(0.0);
+      epi_add(h, 1.0, c->yth);
+      RUN(combine(c, c->Ng, c->Rth, h));
+      RUN(implicit_solve(c, c->Rq, c->Rw, c->Rth, c->yq, c->yw, c->yth));
+      LK(launch_small_axpy(c->Nq, c->ythb, c->ythb, c->ythb, 0.0, c->Rthb, c->st));
+      LK(launch_small_matvec(c->Nq, c->Nq, c->o.proj_d, c->Rthb, c->ythb, c->st));
+      break;
+    }
+    case NHQG_OP_CORRECTOR:
+      RUN(corrector(c, c->yw, c->yth, c->ythb, c->yth, c->ythb, arg));
+      break;
+    case NHQG_OP_SANITIZE:
+      RUN(sanitize(c));
+      break;
+    default:
+      c->err = "unknown op";
+      return NHQG_ERR_ARG;
+  }
+  CK(hipStreamSynchronize(c->st));
+  return NHQG_OK;
+}
+
+int nhqg_get_aux(nhqg_ctx* c, double* q, double* w, double* th) {
+  if (!c || !q || !w || !th) return NHQG_ERR_ARG;
+  RUN(copy_full_out(c, q, c->Nq, c->E1q));
+  RUN(copy_full_out(c, w, c->Ng, c->E1w));
+  RUN(copy_full_out(c, th, c->Ng, c->E1th));
+  return NHQG_OK;
+}
+
+int nhqg_profile_step(nhqg_ctx* c, double* out, int32_t n_out) {
+  (void)c;
+  (void)out;
+  (void)n_out;
+  return NHQG_ERR_UNSUPPORTED;
+}
+
+const char* nhqg_timing_name(int32_t i) {
+  (void)i;
+  return "";
+}
+
+int nhqg_device_bytes(nhqg_ctx* c, uint64_t* bytes) {
+  if (!c || !bytes) return NHQG_ERR_ARG;
+  *bytes = c->bytes;
+  return NHQG_OK;
+}
+
+const char* nhqg_last_error(const nhqg_ctx* c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+}  // extern "C"

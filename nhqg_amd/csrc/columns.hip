@@ -1,0 +1,456 @@
+// Per-mode and per-level kernels of the NHQG step: the batched |k|^2-shell
+// w solve, deterministic Parseval horizontal means, the SBP2 thermal
+// corrector's serial 1-D pieces, post-step sanitation and the layout
+// conversions at the host boundary.
+#include "common.h"
+
+// ---------------------------------------------------------------------------
+// Shell solve (nhqg/solver.py:648-675, 720-736): for every active |k|^2
+// shell s, w = inv_s * rhs for all modes of the shell, fused with the theta
+// back-substitution theta = (R_th + alpha * w) * inv_alpha_th.
+// invT holds each shell inverse transposed ([s][j][i]) so that the threads
+// of a block (one output row each) read consecutive addresses.  The kernel
+// is bound by streaming the (Nz-1)^2 matrices once per call.
+// ---------------------------------------------------------------------------
+#define SH_CH 8
+
+__global__ void __launch_bounds__(256)
+shell_solve_kernel(int Ng, int ld, const int* __restrict__ shell_ptr,
+                   const int* __restrict__ shell_modes, const double* __restrict__ invT,
+                   const double* __restrict__ rhs, double* __restrict__ w_out,
+                   const double* __restrict__ Rth, double* __restrict__ th_out,
+                   const double* __restrict__ iath, double alpha) {
+  extern __shared__ double rs[];  // [Ng][2*SH_CH]
+  __shared__ int ms[SH_CH];
+  const int s = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int p0 = shell_ptr[s], p1 = shell_ptr[s + 1];
+  const double* M = invT + (size_t)s * Ng * Ng;
+  for (int c0 = p0; c0 < p1; c0 += SH_CH) {
+    const int nc = min(SH_CH, p1 - c0);
+    if (tid < SH_CH) ms[tid] = tid < nc ? shell_modes[c0 + tid] : -1;
+    __syncthreads();
+    for (int idx = tid; idx < Ng * 2 * SH_CH; idx += 256) {
+      const int j = idx / (2 * SH_CH), c = idx % (2 * SH_CH);
+      const int m = ms[c >> 1];
+      rs[idx] = m >= 0 ? rhs[(size_t)j * ld + 2 * m + (c & 1)] : 0.0;
+    }
+    __syncthreads();
+    for (int i = tid; i < Ng; i += 256) {
+      double acc[2 * SH_CH];
+#pragma unroll
+      for (int c = 0; c < 2 * SH_CH; ++c) acc[c] = 0.0;
+#pragma unroll 4
+      for (int j = 0; j < Ng; ++j) {
+        const double a = M[(size_t)j * Ng + i];
+        const double* r = rs + j * 2 * SH_CH;
+#pragma unroll
+        for (int c = 0; c < 2 * SH_CH; ++c) acc[c] = fma(a, r[c], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 2 * SH_CH; ++c) {
+        const int m = ms[c >> 1];
+        if (m >= 0) {
+          const size_t idx = (size_t)i * ld + 2 * m + (c & 1);
+          w_out[idx] = acc[c];
+          th_out[idx] = (Rth[idx] + alpha * acc[c]) * iath[m];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_shell_solve(int n_shells, int Ng, int ld, const int* shell_ptr,
+                              const int* shell_modes, const double* invT, const double* rhs,
+                              double* w_out, const double* Rth, double* th_out,
+                              const double* iath, double alpha, hipStream_t s) {
+  const size_t shm = (size_t)Ng * 2 * SH_CH * sizeof(double);
+  hipLaunchKernelGGL(shell_solve_kernel, dim3(n_shells), dim3(256), shm, s, Ng, ld, shell_ptr,
+                     shell_modes, invT, rhs, w_out, Rth, th_out, iath, alpha);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Horizontal means <a b>_xy per level by discrete Parseval:
+//   sum_k pw_k Re(Ha_k conj(Hb_k)),   pw = {1 on the ky=0 and ky=Nx/2
+//   columns, 2 elsewhere} / Nx^4,
+// where H is the Hermitian part on the self-conjugate columns (what irfft2
+// keeps).  This equals mean(irfft2(a) * irfft2(b)) of
+// nhqg/solver.py:231-245 exactly in exact arithmetic; the products are
+// summed in a fixed tree (no atomics), partials[l][p] per block.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+parseval_partial_kernel(int nm, int ld, int P, const double* __restrict__ a,
+                        const double* __restrict__ b, const double* __restrict__ pw,
+                        const int* __restrict__ partner, double* __restrict__ partial) {
+  __shared__ double red[256];
+  const int l = blockIdx.y, p = blockIdx.x, tid = threadIdx.x;
+  const int chunk = (nm + P - 1) / P;
+  const int m0 = p * chunk, m1 = min(nm, m0 + chunk);
+  const double* ar = a + (size_t)l * ld;
+  const double* br = b + (size_t)l * ld;
+  double sum = 0.0;
+  for (int m = m0 + tid; m < m1; m += 256) {
+    double xr = ar[2 * m], xi = ar[2 * m + 1];
+    double yr = br[2 * m], yi = br[2 * m + 1];
+    const int q = partner ? partner[m] : -1;
+    if (q >= 0) {
+      xr = 0.5 * (xr + ar[2 * q]);
+      xi = 0.5 * (xi - ar[2 * q + 1]);
+      yr = 0.5 * (yr + br[2 * q]);
+      yi = 0.5 * (yi - br[2 * q + 1]);
+    }
+    sum += pw[m] * (xr * yr + xi * yi);
+  }
+  red[tid] = sum;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) partial[(size_t)l * P + p] = red[0];
+}
+
+hipError_t launch_parseval(int nl, int nm, int ld, int P, const double* a, const double* b,
+                           const double* pw, const int* partner, double* partial,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(parseval_partial_kernel, dim3(P, nl), dim3(256), 0, s, nm, ld, P, a, b, pw,
+                     partner, partial);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// SBP2 corrector, serial 1-D part (nhqg/solver.py:929-942, 885-905).
+// Banded storage: band[i*5 + (j-i+2)] for |j-i| <= 2; d1b/lb hold the
+// tridiagonal SBP D1 and Laplacian as [i*3 + (j-i+1)].
+// sbp buffer layout (doubles, n = Nz+1):
+//   [0,5n) LU of A (multipliers below the diagonal), [5n,10n) B,
+//   [10n,11n) <w^2>, [11n,12n) flux, [12n,13n) thb_sbp, [13n,14n) rhs/new,
+//   [14n,15n) g
+// ---------------------------------------------------------------------------
+#define SBP_LU 0
+#define SBP_B 5
+#define SBP_W2 10
+#define SBP_FL 11
+#define SBP_TB 12
+#define SBP_NEW 13
+#define SBP_G 14
+
+NHQG_DEV double band3(const double* b3, int i, int j, int n) {
+  const int o = j - i;
+  if (o < -1 || o > 1 || j < 0 || j >= n) return 0.0;
+  return b3[i * 3 + o + 1];
+}
+
+__global__ void __launch_bounds__(256)
+sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
+                   const double* __restrict__ lb, double c1, double c2, double* __restrict__ sbp) {
+  const int tid = threadIdx.x;
+  double* LU = sbp + SBP_LU * n;
+  double* B = sbp + SBP_B * n;
+  double* w2 = sbp + SBP_W2 * n;
+  for (int i = tid; i < n; i += 256) {
+    double s = 0.0;
+    for (int p = 0; p < P; ++p) s += partial[(size_t)i * P + p];
+    w2[i] = s;
+  }
+  __syncthreads();
+  // half = c1 * L + c2 * D1 diag(w2) D1 (pentadiagonal)
+  for (int i = tid; i < n; i += 256) {
+    for (int o = -2; o <= 2; ++o) {
+      const int j = i + o;
+      double h = 0.0;
+      if (j >= 0 && j < n) {
+        h = c1 * band3(lb, i, j, n);
+        for (int k = i - 1; k <= i + 1; ++k) {
+          if (k < 0 || k >= n) continue;
+          h += c2 * band3(d1b, i, k, n) * w2[k] * band3(d1b, k, j, n);
+        }
+      }
+      const double id = (o == 0) ? 1.0 : 0.0;
+      B[i * 5 + o + 2] = id + h;
+      LU[i * 5 + o + 2] = (i == 0 || i == n - 1) ? id : id - h;
+    }
+  }
+  __syncthreads();
+  // Banded LU without pivoting: A = I + (positive semidefinite, diagonally
+  // dominant band) with identity boundary rows, so elimination is stable.
+  if (tid == 0) {
+    for (int k = 0; k < n - 1; ++k) {
+      const double piv = LU[k * 5 + 2];
+      for (int i = k + 1; i <= min(k + 2, n - 1); ++i) {
+        const double f = LU[i * 5 + (k - i) + 2] / piv;
+        LU[i * 5 + (k - i) + 2] = f;
+        for (int j = k + 1; j <= min(k + 2, n - 1); ++j) LU[i * 5 + (j - i) + 2] -= f * LU[k * 5 + (j - k) + 2];
+      }
+    }
+  }
+}
+
+// One substep's 1-D solve: flux -> rhs -> thb_sbp_new -> g, thb.
+__global__ void __launch_bounds__(256)
+sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
+                   const double* __restrict__ thb2s, const double* __restrict__ s2thb,
+                   double mu_sdt, double* __restrict__ sbp, double* __restrict__ thb_io,
+                   double* __restrict__ g_out) {
+  const int tid = threadIdx.x;
+  const double* LU = sbp + SBP_LU * n;
+  const double* B = sbp + SBP_B * n;
+  double* fl = sbp + SBP_FL * n;
+  double* tb = sbp + SBP_TB * n;
+  double* nw = sbp + SBP_NEW * n;
+  for (int i = tid; i < n; i += 256) {
+    double s = 0.0;
+    for (int p = 0; p < P; ++p) s += partial[(size_t)i * P + p];
+    fl[i] = s;
+    double t = 0.0;
+    for (int j = 0; j < n; ++j) t += thb2s[(size_t)i * n + j] * thb_io[j];
+    tb[i] = t;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
+    double r = 0.0;
+    for (int o = -2; o <= 2; ++o) {
+      const int j = i + o;
+      if (j >= 0 && j < n) r += B[i * 5 + o + 2] * tb[j];
+    }
+    double d = 0.0;
+    for (int o = -1; o <= 1; ++o) d += band3(d1b, i, i + o, n) * ((i + o >= 0 && i + o < n) ? fl[i + o] : 0.0);
+    r -= mu_sdt * d;
+    nw[i] = (i == 0 || i == n - 1) ? 0.0 : r;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int i = 1; i < n; ++i)
+      for (int k = max(0, i - 2); k < i; ++k) nw[i] -= LU[i * 5 + (k - i) + 2] * nw[k];
+    for (int i = n - 1; i >= 0; --i) {
+      double v = nw[i];
+      for (int j = i + 1; j <= min(i + 2, n - 1); ++j) v -= LU[i * 5 + (j - i) + 2] * nw[j];
+      nw[i] = v / LU[i * 5 + 2];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
+    double gsum = 0.0;
+    for (int o = -1; o <= 1; ++o) {
+      const int j = i + o;
+      if (j >= 0 && j < n) gsum += band3(d1b, i, j, n) * (tb[j] + nw[j]);
+    }
+    g_out[i] = 0.5 * gsum;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    nw[0] = 0.0;
+    nw[n - 1] = 0.0;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
+    double t = 0.0;
+    for (int j = 0; j < n; ++j) t += s2thb[(size_t)i * n + j] * nw[j];
+    thb_io[i] = t;
+  }
+}
+
+hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double* d1b,
+                              const double* lb, double c1, double c2, double* sbp,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(sbp_prepare_kernel, dim3(1), dim3(256), 0, s, n, P, partial, d1b, lb, c1, c2,
+                     sbp);
+  return hipGetLastError();
+}
+
+hipError_t launch_sbp_substep(int n, int P, const double* partial, const double* d1b,
+                              const double* thb2s, const double* s2thb, double mu_sdt,
+                              double* sbp, double* thb_io, double* g_out, hipStream_t s) {
+  hipLaunchKernelGGL(sbp_substep_kernel, dim3(1), dim3(256), 0, s, n, P, partial, d1b, thb2s,
+                     s2thb, mu_sdt, sbp, thb_io, g_out);
+  return hipGetLastError();
+}
+
+// theta_sbp <- Z0(theta_sbp - sdt * w_sbp * g[l]); rows 0 and n-1 zeroed.
+__global__ void __launch_bounds__(256)
+sbp_update_kernel(size_t total, int ld, int n, double* __restrict__ th, const double* __restrict__ w,
+                  const double* __restrict__ g, double sdt) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= total) return;
+  const int l = (int)(idx / (size_t)ld);
+  th[idx] = (l == 0 || l == n - 1) ? 0.0 : th[idx] - sdt * w[idx] * g[l];
+}
+
+hipError_t launch_sbp_update(int n, int ld, double* th, const double* w, const double* g,
+                             double sdt, hipStream_t s) {
+  const size_t total = (size_t)n * ld;
+  hipLaunchKernelGGL(sbp_update_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     total, ld, n, th, w, g, sdt);
+  return hipGetLastError();
+}
+
+// y = M x for small dense M (n_out x n_in); one block.
+__global__ void __launch_bounds__(256)
+small_matvec_kernel(int n_out, int n_in, const double* __restrict__ M, const double* __restrict__ x,
+                    double* __restrict__ y) {
+  for (int i = threadIdx.x; i < n_out; i += 256) {
+    double t = 0.0;
+    for (int j = 0; j < n_in; ++j) t += M[(size_t)i * n_in + j] * x[j];
+    y[i] = t;
+  }
+}
+
+// y = x0 + c * (x1 - x2) (n entries; one block).
+__global__ void __launch_bounds__(256)
+small_axpy_kernel(int n, const double* x0, const double* x1, const double* x2, double c, double* y) {
+  for (int i = threadIdx.x; i < n; i += 256) y[i] = x0[i] + c * (x1[i] - x2[i]);
+}
+
+hipError_t launch_small_matvec(int n_out, int n_in, const double* M, const double* x, double* y,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(small_matvec_kernel, dim3(1), dim3(256), 0, s, n_out, n_in, M, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_small_axpy(int n, const double* x0, const double* x1, const double* x2, double c,
+                             double* y, hipStream_t s) {
+  hipLaunchKernelGGL(small_axpy_kernel, dim3(1), dim3(256), 0, s, n, x0, x1, x2, c, y);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Post-step sanitation (nhqg/solver.py:98-146, 993-999): zero the k=0 mode,
+// Hermitian-project the self-conjugate columns pairwise, and (full layout
+// only) multiply by the 2/3 mask.  pairs[p] = (m, m') with m <= m'.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+sanitize_kernel(int nl, int ld, int npairs, const int2* __restrict__ pairs, int m0,
+                double* __restrict__ f) {
+  const int l = blockIdx.y;
+  double* r = f + (size_t)l * ld;
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < npairs; p += gridDim.x * 256) {
+    const int2 pr = pairs[p];
+    double ax = r[2 * pr.x], ay = r[2 * pr.x + 1];
+    double bx = r[2 * pr.y], by = r[2 * pr.y + 1];
+    if (pr.x == m0) { ax = ay = 0.0; }
+    if (pr.y == m0) { bx = by = 0.0; }
+    if (pr.x == pr.y) {
+      r[2 * pr.x] = 0.5 * (ax + ax);
+      r[2 * pr.x + 1] = 0.5 * (ay - ay);
+    } else {
+      r[2 * pr.x] = 0.5 * (ax + bx);
+      r[2 * pr.x + 1] = 0.5 * (ay - by);
+      r[2 * pr.y] = 0.5 * (bx + ax);
+      r[2 * pr.y + 1] = 0.5 * (by - ay);
+    }
+  }
+}
+
+hipError_t launch_sanitize(int nl, int ld, int npairs, const int2* pairs, int m0, double* f,
+                           hipStream_t s) {
+  const int bx = (npairs + 255) / 256;
+  hipLaunchKernelGGL(sanitize_kernel, dim3(bx > 0 ? bx : 1, nl), dim3(256), 0, s, nl, ld, npairs,
+                     pairs, m0, f);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256)
+mask_kernel(size_t total, int ld, double* f, const double* mask) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= total) return;
+  f[idx] *= mask[(idx % (size_t)ld) >> 1];
+}
+
+hipError_t launch_mask(int nl, int ld, double* f, const double* mask, hipStream_t s) {
+  const size_t total = (size_t)nl * ld;
+  hipLaunchKernelGGL(mask_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, total,
+                     ld, f, mask);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion between the reference's (levels, Nx, Nk) complex arrays
+// and the band-compact layout.  full_idx[m] = row*Nk + ky for kept modes.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+gather_kernel(int nl, int nm, int nm_pad, int full_n, const int* __restrict__ full_idx,
+              const double2* __restrict__ full, double2* __restrict__ comp) {
+  const int l = blockIdx.y;
+  for (int m = blockIdx.x * 256 + threadIdx.x; m < nm_pad; m += gridDim.x * 256)
+    comp[(size_t)l * nm_pad + m] =
+        m < nm ? full[(size_t)l * full_n + full_idx[m]] : make_double2(0.0, 0.0);
+}
+
+__global__ void __launch_bounds__(256)
+zero_kernel(size_t n, double* p) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx < n) p[idx] = 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+scatter_kernel(int nl, int nm, int nm_pad, int full_n, const int* __restrict__ full_idx,
+               const double2* __restrict__ comp, double2* __restrict__ full) {
+  const int l = blockIdx.y;
+  for (int m = blockIdx.x * 256 + threadIdx.x; m < nm; m += gridDim.x * 256)
+    full[(size_t)l * full_n + full_idx[m]] = comp[(size_t)l * nm_pad + m];
+}
+
+hipError_t launch_gather(int nl, int nm, int nm_pad, int full_n, const int* full_idx,
+                         const double2* full, double2* comp, hipStream_t s) {
+  const int bx = (nm_pad + 255) / 256;
+  hipLaunchKernelGGL(gather_kernel, dim3(bx, nl), dim3(256), 0, s, nl, nm, nm_pad, full_n,
+                     full_idx, full, comp);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(int nl, int nm, int nm_pad, int full_n, const int* full_idx,
+                          const double2* comp, double2* full, hipStream_t s) {
+  const size_t nz = (size_t)nl * full_n * 2;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, nz,
+                     (double*)full);
+  const int bx = (nm + 255) / 256;
+  hipLaunchKernelGGL(scatter_kernel, dim3(bx, nl), dim3(256), 0, s, nl, nm, nm_pad, full_n,
+                     full_idx, comp, full);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics reductions.
+// vsum[l][p]: partial sums over modes of pw * w_extra[m] * |Ha|^2 ...
+// (reuse parseval with a per-mode extra weight) and a max reduction.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+max_reduce_kernel(size_t n, const double* __restrict__ x, double* __restrict__ out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 256) m = fmax(m, x[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+hipError_t launch_max_reduce(size_t n, const double* x, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, s, n, x, out);
+  return hipGetLastError();
+}
+
+// E_q += i beta kx inv_denom q  (the -i beta kx psi term, psi = -q inv_denom;
+// nhqg/solver.py:547).
+__global__ void __launch_bounds__(256)
+beta_kernel(int nm, int ldc, double beta, const double* __restrict__ kx_mode,
+            const double* __restrict__ inv_denom, const double2* __restrict__ q,
+            double2* __restrict__ eq) {
+  const int l = blockIdx.y;
+  for (int m = blockIdx.x * 256 + threadIdx.x; m < nm; m += gridDim.x * 256) {
+    const size_t idx = (size_t)l * ldc + m;
+    eq[idx] = cadd(eq[idx], cmul_i(q[idx], beta * kx_mode[m] * inv_denom[m]));
+  }
+}
+
+hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mode,
+                       const double* inv_denom, const double2* q, double2* eq, hipStream_t s) {
+  hipLaunchKernelGGL(beta_kernel, dim3((nm + 255) / 256, nl), dim3(256), 0, s, nm, ldc, beta,
+                     kx_mode, inv_denom, q, eq);
+  return hipGetLastError();
+}

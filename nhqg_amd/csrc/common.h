@@ -1,0 +1,127 @@
+// Shared definitions for the NHQG gfx950 kernels.
+//
+// Field layout in HBM (band-compact, level-major): a field with nl vertical
+// levels is a row-major real matrix of nl rows and ld = 2*nm_pad columns;
+// column 2m / 2m+1 hold Re / Im of mode m.  Modes are ordered ky-major,
+// m = ky_i * nkx + kx_i, over the kept rows (|kx| <= Nx/3 for the 2/3 band,
+// or all rows) and kept ky columns; nm_pad rounds the mode count up to a
+// multiple of 32 so every contraction tile is full.  Padding modes carry
+// zeros and zero per-mode factors.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define NHQG_DEV __device__ __forceinline__
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+NHQG_DEV double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+NHQG_DEV double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+NHQG_DEV double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+NHQG_DEV double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+NHQG_DEV double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// multiply by i*s
+NHQG_DEV double2 cmul_i(double2 a, double s) { return make_double2(-a.y * s, a.x * s); }
+
+// Linear epilogue shared by the contraction and the elementwise kernels:
+//   out = a_acc * s_acc[mode] * acc + sum_t aX[t] * sX[t][mode] * X[t][idx]
+//   out *= s_out[mode]                       (when s_out is set)
+// A null per-mode pointer means a factor of one; X terms with null
+// pointers are skipped.  idx is the element index in the output field.
+#define NHQG_EPI_TERMS 8
+struct Epi {
+  double a_acc;
+  const double* s_acc;
+  int nX;
+  const double* X[NHQG_EPI_TERMS];
+  double aX[NHQG_EPI_TERMS];
+  const double* sX[NHQG_EPI_TERMS];
+  const double* s_out;
+};
+
+NHQG_DEV double epi_apply(const Epi& e, double acc, size_t idx, int mode) {
+  double o = e.a_acc * acc;
+  if (e.s_acc) o *= e.s_acc[mode];
+  for (int t = 0; t < e.nX; ++t) {
+    double v = e.aX[t] * e.X[t][idx];
+    if (e.sX[t]) v *= e.sX[t][mode];
+    o += v;
+  }
+  if (e.s_out) o *= e.s_out[mode];
+  return o;
+}
+
+static inline Epi epi_make(double a_acc) {
+  Epi e;
+  e.a_acc = a_acc;
+  e.s_acc = nullptr;
+  e.nX = 0;
+  for (int t = 0; t < NHQG_EPI_TERMS; ++t) {
+    e.X[t] = nullptr;
+    e.aX[t] = 0.0;
+    e.sX[t] = nullptr;
+  }
+  e.s_out = nullptr;
+  return e;
+}
+
+static inline void epi_add(Epi& e, double a, const double* X, const double* s = nullptr) {
+  e.X[e.nX] = X;
+  e.aX[e.nX] = a;
+  e.sX[e.nX] = s;
+  e.nX++;
+}
+
+// ---- launch wrappers (defined in the .hip files) ----
+struct FluxArgs {
+  int N, nq, nkx, nky, ldc;
+  const double2 *qn, *wn, *thn;
+  double2 *aq, *aw, *ath;
+  const int *kidx_of_x, *rows;
+  const double *kx_row, *ky_col, *inv_denom, *mask, *eta_phys, *sponge_phys;
+  const double2* tw;
+  double2 *X5, *Y6;
+  size_t fstride;
+};
+hipError_t launch_flux(const FluxArgs& a, hipStream_t s);
+hipError_t launch_speed(const FluxArgs& a, double* rowmax, hipStream_t s);
+hipError_t launch_shell_solve(int n_shells, int Ng, int ld, const int* shell_ptr,
+                              const int* shell_modes, const double* invT, const double* rhs,
+                              double* w_out, const double* Rth, double* th_out,
+                              const double* iath, double alpha, hipStream_t s);
+hipError_t launch_parseval(int nl, int nm, int ld, int P, const double* a, const double* b,
+                           const double* pw, const int* partner, double* partial,
+                           hipStream_t s);
+hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double* d1b,
+                              const double* lb, double c1, double c2, double* sbp,
+                              hipStream_t s);
+hipError_t launch_sbp_substep(int n, int P, const double* partial, const double* d1b,
+                              const double* thb2s, const double* s2thb, double mu_sdt,
+                              double* sbp, double* thb_io, double* g_out, hipStream_t s);
+hipError_t launch_sbp_update(int n, int ld, double* th, const double* w, const double* g,
+                             double sdt, hipStream_t s);
+hipError_t launch_small_matvec(int n_out, int n_in, const double* M, const double* x, double* y,
+                               hipStream_t s);
+hipError_t launch_small_axpy(int n, const double* x0, const double* x1, const double* x2, double c,
+                             double* y, hipStream_t s);
+hipError_t launch_sanitize(int nl, int ld, int npairs, const int2* pairs, int m0, double* f,
+                           hipStream_t s);
+hipError_t launch_mask(int nl, int ld, double* f, const double* mask, hipStream_t s);
+hipError_t launch_gather(int nl, int nm, int nm_pad, int full_n, const int* full_idx,
+                         const double2* full, double2* comp, hipStream_t s);
+hipError_t launch_scatter(int nl, int nm, int nm_pad, int full_n, const int* full_idx,
+                          const double2* comp, double2* full, hipStream_t s);
+hipError_t launch_max_reduce(size_t n, const double* x, double* out, hipStream_t s);
+hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mode,
+                       const double* inv_denom, const double2* q, double2* eq, hipStream_t s);
+
+// C[M x N] = epilogue(A[M x K] * B[K x N]); B, C and the epilogue X fields
+// share the row stride ld (doubles); N must be a multiple of 64.
+hipError_t launch_gemm_f64(int M, int N, int K, const double* A, int lda, const double* B,
+                           double* C, int ld, const Epi& e, hipStream_t s);
+// out[l][c] = epilogue(0) over nl rows of ld doubles.
+hipError_t launch_combine(int nl, int ld, double* out, const Epi& e, hipStream_t s);

@@ -1,0 +1,446 @@
+// Horizontal spectral products of the NHQG explicit tendency, 2/3 rule.
+//
+// The reference evaluates, per vertical level, div(u f, v f) with
+// u = -psi_y, v = psi_x for f in {q'+eta, w, theta} using irfft2 / rfft2 on
+// the Nx grid and masks the result (nhqg/spectral.py:307-328), and adds the
+// sponge products rfft2(sigma * irfft2(f)) (spectral.py:53-60).  Here the
+// whole per-level chain is three fused passes over LDS-resident Stockham
+// FFTs (radix 4, one radix-2 stage when log2 Nx is odd; twiddles from a
+// host table, no recursion):
+//
+//   col_inverse : per (level, ky): inverse DFT along x of
+//                 U = i ky psi, V = -i kx psi, Q, W, Theta (psi = -q / |k|^2
+//                 is formed on the fly from the nodal q)           -> X5
+//   row_flux    : per (level, x):  packed c2r along y (two real rows per
+//                 complex FFT), pointwise products u(q+eta), v(q+eta), uw,
+//                 vw, u theta, v theta, sigma q, sigma w, sigma theta,
+//                 packed r2c along y; applies i ky and the sponge    -> Y6
+//   col_forward : per (level, ky): forward DFT along x, i kx G + H,
+//                 gather of the kept rows (2/3 band)                -> A
+//
+// A "team" of Nx/4 threads owns one transform; a 256-thread block holds
+// 1024/Nx teams.  Sponge and trap are folded into the same passes: the
+// sponge acts on nodal fields, which equals acting on coefficient levels
+// because sigma is independent of Z and the vertical maps are linear.
+#include "common.h"
+
+template <int N> struct Log2;
+template <> struct Log2<16> { static constexpr int v = 4; };
+template <> struct Log2<32> { static constexpr int v = 5; };
+template <> struct Log2<64> { static constexpr int v = 6; };
+template <> struct Log2<128> { static constexpr int v = 7; };
+template <> struct Log2<256> { static constexpr int v = 8; };
+template <> struct Log2<512> { static constexpr int v = 9; };
+template <> struct Log2<1024> { static constexpr int v = 10; };
+
+template <bool INV>
+NHQG_DEV double2 twid(const double2* tw, int e) {
+  const double2 w = tw[e];
+  return INV ? make_double2(w.x, -w.y) : w;
+}
+
+// Unnormalised DFT of N points held in LDS buffer a (b is scratch), by a
+// team of N/4 threads with index t.  INV selects exp(+i...).  Every thread
+// of the block must call it (block barriers inside).  Returns the buffer
+// that holds the result in natural order.
+template <int N, bool INV>
+NHQG_DEV double2* team_fft(double2* a, double2* b, int t, const double2* tw) {
+  constexpr int T = N / 4;
+  constexpr int L2 = Log2<N>::v;
+  double2* in = a;
+  double2* out = b;
+  int Ns = 1;
+#pragma unroll
+  for (int s = 0; s < L2 / 2; ++s) {
+    const int j = t;
+    const int k = j & (Ns - 1);
+    double2 v0 = in[j], v1 = in[j + T], v2 = in[j + 2 * T], v3 = in[j + 3 * T];
+    if (s > 0) {
+      const int e = k * (N / (Ns * 4));
+      v1 = cmul(v1, twid<INV>(tw, e));
+      v2 = cmul(v2, twid<INV>(tw, 2 * e));
+      v3 = cmul(v3, twid<INV>(tw, 3 * e));
+    }
+    const double2 a0 = cadd(v0, v2), a1 = csub(v0, v2);
+    const double2 a2 = cadd(v1, v3), a3 = csub(v1, v3);
+    const double2 ia3 = make_double2(-a3.y, a3.x);
+    const int d = (j - k) * 4 + k;
+    out[d] = cadd(a0, a2);
+    out[d + Ns] = INV ? cadd(a1, ia3) : csub(a1, ia3);
+    out[d + 2 * Ns] = csub(a0, a2);
+    out[d + 3 * Ns] = INV ? csub(a1, ia3) : cadd(a1, ia3);
+    __syncthreads();
+    double2* tmp = in;
+    in = out;
+    out = tmp;
+    Ns *= 4;
+  }
+  if (L2 & 1) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = t + h * T;
+      const int k = j & (Ns - 1);
+      const double2 v0 = in[j];
+      const double2 v1 = cmul(in[j + N / 2], twid<INV>(tw, k * (N / (Ns * 2))));
+      const int d = (j - k) * 2 + k;
+      out[d] = cadd(v0, v1);
+      out[d + Ns] = csub(v0, v1);
+    }
+    __syncthreads();
+    double2* tmp = in;
+    in = out;
+    out = tmp;
+  }
+  return in;
+}
+
+template <int N>
+NHQG_DEV void load_twiddles(double2* tw, const double2* twg) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: inverse DFT along x of the five advected inputs.
+// qn/wn/thn: nodal spectral fields (level-major, row stride ldc double2).
+// X5[f][l][x][ky], f = U, V, Q, W, Theta.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256)
+col_inverse_flux(int nkx, int nky, int ldc, const double2* __restrict__ qn,
+                 const double2* __restrict__ wn, const double2* __restrict__ thn,
+                 const int* __restrict__ kidx_of_x, const double* __restrict__ kx_row,
+                 const double* __restrict__ ky_col, const double* __restrict__ inv_denom,
+                 const double2* __restrict__ twg, double2* __restrict__ X5, size_t fstride) {
+  constexpr int T = N / 4, TEAMS = 256 / T;
+  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 tw[N];
+  const int tid = threadIdx.x, team = tid / T, t = tid % T;
+  const int l = blockIdx.y;
+  const int ky0 = blockIdx.x * TEAMS;
+  const int ky = ky0 + team;
+  const bool active = ky < nky;
+  load_twiddles<N>(tw, twg);
+  const size_t lb = (size_t)l * ldc;
+  for (int f = 0; f < 5; ++f) {
+    for (int x = t; x < N; x += T) {
+      double2 v = make_double2(0.0, 0.0);
+      const int i = kidx_of_x[x];
+      if (active && i >= 0) {
+        const int m = ky * nkx + i;
+        if (f < 3) {
+          const double2 qv = qn[lb + m];
+          if (f == 0) v = cmul_i(qv, ky_col[ky] * inv_denom[m]);
+          else if (f == 1) v = cmul_i(qv, -kx_row[i] * inv_denom[m]);
+          else v = qv;
+        } else {
+          v = (f == 3 ? wn : thn)[lb + m];
+        }
+      }
+      buf[0][team][x] = v;
+    }
+    __syncthreads();
+    double2* res = team_fft<N, true>(buf[0][team], buf[1][team], t, tw);
+    const int par = (res == buf[0][team]) ? 0 : 1;
+    double2* dst = X5 + f * fstride + (size_t)l * N * nky;
+    for (int idx = tid; idx < N * TEAMS; idx += 256) {
+      const int cc = idx % TEAMS, x = idx / TEAMS;
+      const int kyc = ky0 + cc;
+      if (kyc < nky) dst[(size_t)x * nky + kyc] = buf[par][cc][x];
+    }
+    __syncthreads();
+  }
+}
+
+// Build the packed spectrum Z of (p + i q) along y from two half spectra
+// (ky < nky) so that its inverse DFT is p(y) + i q(y), with irfft's
+// convention that only the real parts of the ky=0 and ky=N/2 entries count.
+template <int N>
+NHQG_DEV void build_packed(double2* z, int t, int nky, const double2* P, const double2* Q) {
+  constexpr int T = N / 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = t + j * T;
+    const int ky = (k <= N / 2) ? k : N - k;
+    double2 v = make_double2(0.0, 0.0);
+    if (ky < nky) {
+      const double2 p = P[ky];
+      const double2 q = Q ? Q[ky] : make_double2(0.0, 0.0);
+      if (k == 0 || k == N / 2) v = make_double2(p.x, q.x);
+      else if (k < N / 2) v = make_double2(p.x - q.y, p.y + q.x);
+      else v = make_double2(p.x + q.y, q.x - p.y);
+    }
+    z[k] = v;
+  }
+}
+
+// Forward DFT along y of two real rows (a, b) held as 4 values per thread
+// at y = t + j*N/4; returns half-spectrum slots ky = t + s*N/4 (s < 3).
+template <int N>
+NHQG_DEV void fwd_pair(double2* ba, double2* bb, int t, const double2* tw, const double* a,
+                       const double* b, int nky, double2* A, double2* B) {
+  constexpr int T = N / 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) ba[t + j * T] = make_double2(a[j], b ? b[j] : 0.0);
+  __syncthreads();
+  const double2* r = team_fft<N, false>(ba, bb, t, tw);
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int ky = t + s * T;
+    if (ky < nky) {
+      const double2 zk = r[ky];
+      const double2 zm = r[(N - ky) & (N - 1)];
+      A[s] = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+      B[s] = make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
+    } else {
+      A[s] = make_double2(0.0, 0.0);
+      B[s] = make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+}
+
+template <int N>
+NHQG_DEV void inv_pair(double2* ba, double2* bb, int t, const double2* tw, int nky,
+                       const double2* P, const double2* Q, double* p, double* q, double scale) {
+  constexpr int T = N / 4;
+  build_packed<N>(ba, t, nky, P, Q);
+  __syncthreads();
+  const double2* r = team_fft<N, true>(ba, bb, t, tw);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double2 z = r[t + j * T];
+    p[j] = z.x * scale;
+    if (q) q[j] = z.y * scale;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2: per (level, x) row.  Y6[g][l][x][ky], g = Gq, Hq, Gw, Hw, Gth, Hth
+// with G_f = F_y(u f) and H_f = i ky F_y(v f) + F_y(sigma f).
+// ---------------------------------------------------------------------------
+template <int N, bool SPONGE, bool TRAP>
+__global__ void __launch_bounds__(256)
+row_flux(int nky, const double2* __restrict__ X5, size_t fstride, double2* __restrict__ Y6,
+         const double* __restrict__ ky_col, const double* __restrict__ eta_phys,
+         const double* __restrict__ sponge_phys, const double2* __restrict__ twg) {
+  constexpr int T = N / 4, TEAMS = 256 / T;
+  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 tw[N];
+  const int tid = threadIdx.x, team = tid / T, t = tid % T;
+  const int l = blockIdx.y;
+  int x = blockIdx.x * TEAMS + team;
+  const bool active = x < N;
+  if (!active) x = N - 1;  // duplicate a valid row; results are not stored
+  load_twiddles<N>(tw, twg);
+  __syncthreads();
+  double2* ba = buf[0][team];
+  double2* bb = buf[1][team];
+  const size_t ro = ((size_t)l * N + x) * nky;
+  const double sc = 1.0 / ((double)N * (double)N);
+  double u[4], v[4], q[4], w[4], th[4];
+  inv_pair<N>(ba, bb, t, tw, nky, X5 + 0 * fstride + ro, X5 + 1 * fstride + ro, u, v, sc);
+  inv_pair<N>(ba, bb, t, tw, nky, X5 + 2 * fstride + ro, X5 + 3 * fstride + ro, q, w, sc);
+  inv_pair<N>(ba, bb, t, tw, nky, X5 + 4 * fstride + ro, nullptr, th, nullptr, sc);
+
+  double2 Sq[3], Sw[3], St[3];
+  for (int s = 0; s < 3; ++s) Sq[s] = Sw[s] = St[s] = make_double2(0.0, 0.0);
+  if (SPONGE) {
+    double a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double sg = sponge_phys[(size_t)x * N + t + j * T];
+      a[j] = sg * q[j];
+      b[j] = sg * w[j];
+    }
+    fwd_pair<N>(ba, bb, t, tw, a, b, nky, Sq, Sw);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = sponge_phys[(size_t)x * N + t + j * T] * th[j];
+    double2 dummy[3];
+    fwd_pair<N>(ba, bb, t, tw, a, nullptr, nky, St, dummy);
+  }
+  if (TRAP) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) q[j] += eta_phys[(size_t)x * N + t + j * T];
+  }
+  const double* fld[3] = {q, w, th};
+  const double2* S[3] = {Sq, Sw, St};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    double a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a[j] = u[j] * fld[f][j];
+      b[j] = v[j] * fld[f][j];
+    }
+    double2 G[3], Hv[3];
+    fwd_pair<N>(ba, bb, t, tw, a, b, nky, G, Hv);
+    if (active) {
+      double2* dg = Y6 + (2 * f) * fstride + ro;
+      double2* dh = Y6 + (2 * f + 1) * fstride + ro;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int ky = t + s * T;
+        if (ky < nky) {
+          dg[ky] = G[s];
+          dh[ky] = cadd(cmul_i(Hv[s], ky_col[ky]), S[f][s]);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: forward DFT along x; A_f(kx, ky) = i kx F_x(G_f) + F_x(H_f) at the
+// kept rows, times the output mask.  Writes A into the nodal buffers.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256)
+col_forward_flux(int nkx, int nky, int ldc, const double2* __restrict__ Y6, size_t fstride,
+                 const int* __restrict__ rows, const double* __restrict__ kx_row,
+                 const double* __restrict__ mask, double2* __restrict__ out_q,
+                 double2* __restrict__ out_w, double2* __restrict__ out_th,
+                 const double2* __restrict__ twg) {
+  constexpr int T = N / 4, TEAMS = 256 / T;
+  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 tw[N];
+  const int tid = threadIdx.x, team = tid / T, t = tid % T;
+  const int l = blockIdx.y;
+  const int ky0 = blockIdx.x * TEAMS;
+  const int ky = ky0 + team;
+  const bool active = ky < nky;
+  load_twiddles<N>(tw, twg);
+  double2* outs[3] = {out_q, out_w, out_th};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    double2 g[4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const double2* src = Y6 + (2 * f + h) * fstride + (size_t)l * N * nky;
+      for (int idx = tid; idx < N * TEAMS; idx += 256) {
+        const int cc = idx % TEAMS, x = idx / TEAMS;
+        const int kyc = ky0 + cc;
+        buf[0][cc][x] = (kyc < nky) ? src[(size_t)x * nky + kyc] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+      const double2* r = team_fft<N, false>(buf[0][team], buf[1][team], t, tw);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = t + j * T;
+        if (i < nkx) {
+          const double2 z = r[rows[i]];
+          if (h == 0) {
+            g[j] = cmul_i(z, kx_row[i]);
+          } else if (active) {
+            const int m = ky * nkx + i;
+            outs[f][(size_t)l * ldc + m] = cscale(cadd(g[j], z), mask[m]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity magnitude for the max_speed diagnostic: per (level, x) row,
+// c2r of U = i ky psi, V = -i kx psi (from X5 slots 0, 1) and a row max of
+// sqrt(u^2 + v^2).  rowmax[l*N + x].
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256)
+row_speed(int nky, const double2* __restrict__ X5, size_t fstride, double* __restrict__ rowmax,
+          const double2* __restrict__ twg) {
+  constexpr int T = N / 4, TEAMS = 256 / T;
+  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 tw[N];
+  __shared__ double red[256];
+  const int tid = threadIdx.x, team = tid / T, t = tid % T;
+  const int l = blockIdx.y;
+  int x = blockIdx.x * TEAMS + team;
+  const bool active = x < N;
+  if (!active) x = N - 1;
+  load_twiddles<N>(tw, twg);
+  __syncthreads();
+  const size_t ro = ((size_t)l * N + x) * nky;
+  const double sc = 1.0 / ((double)N * (double)N);
+  double u[4], v[4];
+  inv_pair<N>(buf[0][team], buf[1][team], t, tw, nky, X5 + ro, X5 + fstride + ro, u, v, sc);
+  double mx = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) mx = fmax(mx, sqrt(u[j] * u[j] + v[j] * v[j]));
+  red[tid] = mx;
+  __syncthreads();
+  if (t == 0 && active) {
+    double r = red[tid];
+    for (int i = 1; i < T; ++i) r = fmax(r, red[tid + i]);
+    rowmax[(size_t)l * N + x] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+
+
+template <int N>
+static hipError_t flux_n(const FluxArgs& a, hipStream_t s) {
+  constexpr int TEAMS = 1024 / N;
+  dim3 gc((a.nky + TEAMS - 1) / TEAMS, a.nq);
+  hipLaunchKernelGGL(col_inverse_flux<N>, gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.qn, a.wn,
+                     a.thn, a.kidx_of_x, a.kx_row, a.ky_col, a.inv_denom, a.tw, a.X5, a.fstride);
+  dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
+  const bool sp = a.sponge_phys != nullptr, tr = a.eta_phys != nullptr;
+  if (sp && tr)
+    hipLaunchKernelGGL((row_flux<N, true, true>), gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
+                       a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+  else if (sp)
+    hipLaunchKernelGGL((row_flux<N, true, false>), gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
+                       a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+  else if (tr)
+    hipLaunchKernelGGL((row_flux<N, false, true>), gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
+                       a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+  else
+    hipLaunchKernelGGL((row_flux<N, false, false>), gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
+                       a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+  hipLaunchKernelGGL(col_forward_flux<N>, gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.Y6,
+                     a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath, a.tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_flux(const FluxArgs& a, hipStream_t s) {
+  switch (a.N) {
+    case 16: return flux_n<16>(a, s);
+    case 32: return flux_n<32>(a, s);
+    case 64: return flux_n<64>(a, s);
+    case 128: return flux_n<128>(a, s);
+    case 256: return flux_n<256>(a, s);
+    case 512: return flux_n<512>(a, s);
+    case 1024: return flux_n<1024>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N>
+static hipError_t speed_n(const FluxArgs& a, double* rowmax, hipStream_t s) {
+  constexpr int TEAMS = 1024 / N;
+  dim3 gc((a.nky + TEAMS - 1) / TEAMS, a.nq);
+  hipLaunchKernelGGL(col_inverse_flux<N>, gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.qn, a.qn,
+                     a.qn, a.kidx_of_x, a.kx_row, a.ky_col, a.inv_denom, a.tw, a.X5, a.fstride);
+  dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
+  hipLaunchKernelGGL(row_speed<N>, gr, dim3(256), 0, s, a.nky, a.X5, a.fstride, rowmax, a.tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_speed(const FluxArgs& a, double* rowmax, hipStream_t s) {
+  switch (a.N) {
+    case 16: return speed_n<16>(a, rowmax, s);
+    case 32: return speed_n<32>(a, rowmax, s);
+    case 64: return speed_n<64>(a, rowmax, s);
+    case 128: return speed_n<128>(a, rowmax, s);
+    case 256: return speed_n<256>(a, rowmax, s);
+    case 512: return speed_n<512>(a, rowmax, s);
+    case 1024: return speed_n<1024>(a, rowmax, s);
+    default: return hipErrorInvalidValue;
+  }
+}

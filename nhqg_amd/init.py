@@ -1,0 +1,117 @@
+"""Seeded initial states for the benchmark workloads.
+
+Patterns follow the reference drivers: a random thermal perturbation with a
+sin(pi Z) envelope (nhqg/solver.py:1359-1361) and a barotropic (T_0
+coefficient) vorticity annulus scaled to a target rms velocity
+(scripts/run_polar.py:115-158).  numpy's default_rng makes them
+reproducible.
+
+thermal_noise_state: white noise -> rfft2 (Hermitian by construction) ->
+amplitude filter k exp(-k^2 / (2 k_c^2)) (power spectrum proportional to
+k^2 exp(-k^2/k_c^2)) -> 2/3 mask -> horizontal pattern of rms
+``amplitude`` -> times sin(pi Z) at the CGL nodes -> Chebyshev coefficients
+-> Dirichlet tau projection -> Galerkin coefficients.  w = q' = 0.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from nhqg_amd.grid import Grid
+from nhqg_amd.solver import State, sanitize_state
+
+K_ONSET = 1.3048  # onset wavenumber k_c of the stress-free problem
+
+
+def _weights(grid: Grid) -> np.ndarray:
+    w = np.full(grid.Nk, 2.0)
+    w[0] = 1.0
+    if grid.Nx % 2 == 0:
+        w[-1] = 1.0
+    return w[None, :]
+
+
+def thermal_noise_state(grid: Grid, seed: int = 0, amplitude: float = 1e-3,
+                        k_c: float = K_ONSET) -> State:
+    Nx = grid.Nx
+    rng = np.random.default_rng(seed)
+    nh = np.fft.rfft2(rng.standard_normal((Nx, Nx)))
+    filt = np.sqrt(grid.ksq) * np.exp(-grid.ksq / (2.0 * k_c ** 2))
+    pat = np.fft.irfft2(nh * filt * grid.mask_23, s=(Nx, Nx))
+    pat *= amplitude / pat.std()
+    ph = np.fft.rfft2(pat) * grid.mask_23
+    nodal = np.sin(np.pi * grid.Z)[:, None, None] * ph[None]
+    cheb = np.tensordot(grid.proj_dirichlet @ grid.V_inv, nodal, axes=1)
+    th = np.tensordot(grid.dirichlet_pinv, cheb, axes=1)
+    Nq, Ng = grid.Nz + 1, grid.Nz - 1
+    q = np.zeros((Nq, Nx, grid.Nk), np.complex128)
+    w = np.zeros((Ng, Nx, grid.Nk), np.complex128)
+    return sanitize_state(State(q, w, th, np.zeros(Nq)), grid)
+
+
+def rms_velocity(q0: np.ndarray, grid: Grid) -> float:
+    """rms of (u, v) = (-psi_y, psi_x) for one level's q spectrum."""
+    psi = -q0 * grid.inv_denom
+    e = np.sum(_weights(grid) * grid.ksq * np.abs(psi) ** 2) / float(grid.Nx) ** 4
+    return float(np.sqrt(e))
+
+
+def add_barotropic_flow(state: State, grid: Grid, seed: int = 1, u_rms: float = 15.0,
+                        k_peak: float | None = None, k_width: float | None = None) -> State:
+    """Add a barotropic vorticity annulus (T_0 coefficient of q') with rms
+    velocity u_rms; k_peak defaults to k_c/4, k_width to k_peak/2."""
+    Nx = grid.Nx
+    k_peak = K_ONSET / 4.0 if k_peak is None else k_peak
+    k_width = 0.5 * k_peak if k_width is None else k_width
+    rng = np.random.default_rng(seed)
+    zh = np.fft.rfft2(rng.standard_normal((Nx, Nx)))
+    k = np.sqrt(grid.ksq)
+    ann = np.exp(-0.5 * ((k - k_peak) / k_width) ** 2)
+    ann[0, 0] = 0.0
+    zh = zh * ann * grid.mask_23
+    zh *= u_rms / rms_velocity(zh, grid)
+    q = np.array(state.q_hat, copy=True)
+    q[0] += zh
+    return sanitize_state(State(q, state.w_hat, state.th_hat, state.th_bar), grid)
+
+
+def polar_config_256(**kw):
+    """Plain FP64 production step on one MI355X: Nx=Ny=256, Nz=256,
+    Ra=100, Pr=1, box 24 onset wavelengths, polar trap gamma=2.2e-3
+    (r* = 0.4 L, A_d = 20), Rayleigh sponge (sigma_max = 50, r_s = 0.45 L,
+    A_s = 30), flux advection, 2/3 rule, evolve_mean + balanced_sbp2_pc with
+    4 corrector substeps, nu = 1, dt = 2.5e-5."""
+    from nhqg_amd.config import NHQGConfig
+
+    L = 24.0 * 2.0 * np.pi / K_ONSET
+    d = dict(Nx=256, Nz=256, L=L, Ra_tilde=100.0, sigma=1.0, gamma=2.2e-3,
+             trap_r_star=0.4 * L, trap_sharpness=20.0, sponge_rate=50.0,
+             sponge_r_start=0.45 * L, sponge_sharpness=30.0, dt=2.5e-5, nu_q=1.0, nu_w=1.0,
+             nu_theta=1.0, hyper_order=1, float_dtype="float64", nonlinear_advection="flux",
+             horizontal_dealiasing="23_rule", thermal_closure="evolve_mean",
+             mean_exchange_discretization="balanced_sbp2_pc", sbp_corrector_substeps=4,
+             w_bc_top="dirichlet")
+    d.update(kw)
+    return NHQGConfig(**d)
+
+
+def cpu_config_64(**kw):
+    """CPU-runnable plain case: Nx=64, Nz=32, box 10 onset wavelengths,
+    Ra=100, no trap, dt=5e-5, evolve_mean + balanced_sbp2_pc (4 substeps)."""
+    from nhqg_amd.config import NHQGConfig
+
+    L = 10.0 * 2.0 * np.pi / K_ONSET
+    d = dict(Nx=64, Nz=32, L=L, Ra_tilde=100.0, sigma=1.0, dt=5e-5, nu_q=1.0, nu_w=1.0,
+             nu_theta=1.0, hyper_order=1, float_dtype="float64", nonlinear_advection="flux",
+             horizontal_dealiasing="23_rule", thermal_closure="evolve_mean",
+             mean_exchange_discretization="balanced_sbp2_pc", sbp_corrector_substeps=4,
+             w_bc_top="dirichlet")
+    d.update(kw)
+    return NHQGConfig(**d)
+
+
+def polar_state_256(grid: Grid, seed: int = 0) -> State:
+    """Seeded Hermitian thermal state of amplitude 1e-2 plus a low-k
+    barotropic flow of rms velocity 15."""
+    s = thermal_noise_state(grid, seed=seed, amplitude=1e-2)
+    return add_barotropic_flow(s, grid, seed=seed + 1, u_rms=15.0)
