@@ -234,6 +234,19 @@ class Context:
         self._check(self.lib.nhqg_diagnostics(self.h, _ptr(out), n), "nhqg_diagnostics")
         return out
 
+    def profile_step(self):
+        """Advance one step eagerly with per-kernel-class HIP event timing.
+        Returns {class: (ms, launches, work)}."""
+        n = 3 * 16
+        out = np.zeros(n, np.float64)
+        self._check(self.lib.nhqg_profile_step(self.h, _ptr(out), n), "nhqg_profile_step")
+        res = {}
+        for i in range(16):
+            name = self.lib.nhqg_timing_name(i).decode()
+            if name:
+                res[name] = (float(out[3 * i]), int(out[3 * i + 1]), float(out[3 * i + 2]))
+        return res
+
     def device_bytes(self):
         v = C.c_uint64()
         self._check(self.lib.nhqg_device_bytes(self.h, C.byref(v)), "nhqg_device_bytes")

@@ -54,11 +54,20 @@ struct nhqg_ctx {
   double *s1q, *s1w, *s1th, *s1thb, *c1th, *c1thb;
   double *rhs_w, *tmpg;
   double *wsbp, *thsbp;
+  double *spq, *spw, *spth;
   double *partial, *sbp, *g;
   double2 *X5, *Y6;
   size_t fstride;
   double2* staging;
   double *diag, *rowmax;
+  // per-launch timing of one eager step (nhqg_profile_step)
+  struct PRec {
+    int cls;
+    hipEvent_t a, b;
+    double work;
+  };
+  bool prof = false;
+  std::vector<PRec> recs;
 };
 
 namespace {
@@ -121,13 +130,53 @@ int field(nhqg_ctx* c, double** p, int nl) {
     }                                                                                 \
   } while (0)
 
+// Kernel classes timed by nhqg_profile_step; work is the class's unit:
+// flops for contractions (4*M*K per complex mode), bytes moved otherwise.
+enum {
+  CL_GEMM = 0,
+  CL_FLUX_COLINV,
+  CL_FLUX_ROW,
+  CL_FLUX_COLFWD,
+  CL_SHELL,
+  CL_PARSEVAL,
+  CL_SBP,
+  CL_ELEMENTWISE,
+  CL_SMALL,
+  CL_COUNT
+};
+const char* const kClassNames[CL_COUNT] = {
+    "gemm_f64_mfma", "flux_col_inverse", "flux_row_products", "flux_col_forward",
+    "shell_solve", "parseval_means", "sbp_serial", "elementwise", "small"};
+
+void prof_begin(nhqg_ctx* c, hipEvent_t* a) {
+  (void)hipEventCreate(a);
+  (void)hipEventRecord(*a, c->st);
+}
+
+void prof_end(nhqg_ctx* c, int cls, hipEvent_t a, double work) {
+  hipEvent_t b = nullptr;
+  (void)hipEventCreate(&b);
+  (void)hipEventRecord(b, c->st);
+  c->recs.push_back({cls, a, b, work});
+}
+
+#define PL(cls, work, expr)                         \
+  do {                                              \
+    hipEvent_t pa_ = nullptr;                       \
+    if (c->prof) prof_begin(c, &pa_);               \
+    LK(expr);                                       \
+    if (c->prof) prof_end(c, (cls), pa_, (work));   \
+  } while (0)
+
 int gemm(nhqg_ctx* c, int M, int K, const double* A, const double* B, double* C, const Epi& e) {
-  LK(launch_gemm_f64(M, c->ld, K, A, K, B, C, c->ld, e, c->st));
+  PL(CL_GEMM, 4.0 * M * K * (double)c->nm,
+     launch_gemm_f64(M, c->ld, K, A, K, B, C, c->ld, e, c->st));
   return NHQG_OK;
 }
 
 int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
-  LK(launch_combine(nl, c->ld, out, e, c->st));
+  PL(CL_ELEMENTWISE, 16.0 * (e.nX + 1) * nl * (double)c->nm,
+     launch_combine(nl, c->ld, out, e, c->st));
   return NHQG_OK;
 }
 
@@ -153,9 +202,16 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   a.qn = (const double2*)c->qn;
   a.wn = (const double2*)c->wn;
   a.thn = (const double2*)c->thn;
+  a.ng = c->Ng;
+  a.qc = (const double2*)q;
+  a.wc = (const double2*)w;
+  a.thc = (const double2*)th;
   a.aq = (double2*)c->qn;
   a.aw = (double2*)c->wn;
   a.ath = (double2*)c->thn;
+  a.sq = (double2*)c->spq;
+  a.sw = (double2*)c->spw;
+  a.sth = (double2*)c->spth;
   a.kidx_of_x = o.kidx_of_x;
   a.rows = o.rows;
   a.kx_row = o.kx_row;
@@ -168,14 +224,31 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   a.X5 = c->X5;
   a.Y6 = c->Y6;
   a.fstride = c->fstride;
-  LK(launch_flux(a, c->st));
-  RUN(gemm(c, c->Nq, c->Nq, o.Vinv, c->qn, Eq, epi_make(-1.0)));
+  const double F = 16.0 * c->Nq * (double)c->Nx * c->nky;   // one intermediate field
+  const double Fc = 16.0 * c->Nq * (double)c->nm;            // one compact field
+  const int sp = a.sponge_phys ? 3 : 0;
+  PL(CL_FLUX_COLINV, (3 + sp) * Fc + (5 + sp) * F, launch_flux(a, 1, c->st));
+  PL(CL_FLUX_ROW, (11 + 2 * sp) * F, launch_flux(a, 2, c->st));
+  PL(CL_FLUX_COLFWD, (6 + sp) * F + (3 + sp) * Fc, launch_flux(a, 3, c->st));
+  // E = -(back transform of the advection) - sponge (solver.py:541-549, 595-597)
+  Epi eq = epi_make(-1.0), ew = epi_make(-1.0), et = epi_make(-1.0);
+  if (a.sponge_phys) {
+    epi_add(eq, -1.0, c->spq);
+    epi_add(ew, -1.0, c->spw);
+    epi_add(et, -1.0, c->spth);
+  }
+  RUN(gemm(c, c->Nq, c->Nq, o.Vinv, c->qn, Eq, eq));
   if (c->P.beta != 0.0)
     LK(launch_beta(c->Nq, c->ldc, c->nm, c->P.beta, o.kx_mode, o.inv_denom, (const double2*)q,
                    (double2*)Eq, c->st));
-  RUN(gemm(c, c->Ng, c->Nq, o.Pw, c->wn, Ew, epi_make(-1.0)));
-  RUN(gemm(c, c->Ng, c->Nq, o.Pd, c->thn, Eth, epi_make(-1.0)));
+  RUN(gemm(c, c->Ng, c->Nq, o.Pw, c->wn, Ew, ew));
+  RUN(gemm(c, c->Ng, c->Nq, o.Pd, c->thn, Eth, et));
   return NHQG_OK;
+}
+
+// matrix stream + rhs, R_th in + w, theta out
+double shell_bytes(const nhqg_ctx* c) {
+  return 8.0 * c->P.n_shells * (double)c->Ng * c->Ng + 4.0 * 16.0 * c->Ng * (double)c->nm;
 }
 
 // imex_implicit_solve (nhqg/solver.py:691-738), q_boundary='none'.
@@ -190,7 +263,8 @@ int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double
     epi_add(e, 1.0, Rw);
     epi_add(e, ars, Rth, o.iath);
     RUN(gemm(c, c->Ng, c->Nq, o.Qw, Rq, c->rhs_w, e));
-    LK(launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
+    PL(CL_SHELL, shell_bytes(c),
+       launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
                           c->rhs_w, ow, Rth, oth, o.iath, al, c->st));
   } else {
     RUN(gemm(c, c->Ng, c->Ng, o.Mth2w, Rth, c->tmpg, epi_make(1.0)));
@@ -200,7 +274,8 @@ int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double
     epi_add(e, ars, c->tmpg, o.iath);
     RUN(gemm(c, c->Ng, c->Nq, o.Qw, Rq, c->rhs_w, e));
     // the fused theta output is overwritten below with the mapped form
-    LK(launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
+    PL(CL_SHELL, shell_bytes(c),
+       launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
                           c->rhs_w, ow, Rth, c->tmpg, o.iath, al, c->st));
     Epi et = epi_make(al);
     epi_add(et, 1.0, Rth);
@@ -224,19 +299,24 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
   const double sdt = total_dt / ns;
   const double mu = c->P.eps_sq, kappa = 1.0 / c->P.sigma;
   RUN(gemm(c, n, c->Ng, o.W2S, w_in, c->wsbp, epi_make(1.0)));
-  LK(launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->wsbp, o.pw, o.partner, c->partial,
+  PL(CL_PARSEVAL, 16.0 * n * (double)c->nm,
+     launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->wsbp, o.pw, o.partner, c->partial,
                      c->st));
-  LK(launch_sbp_prepare(n, c->Pred, c->partial, o.d1b, o.lb, 0.5 * mu * kappa * sdt,
+  PL(CL_SBP, 0.0, launch_sbp_prepare(n, c->Pred, c->partial, o.d1b, o.lb, 0.5 * mu * kappa * sdt,
                         0.25 * mu * sdt * sdt, c->sbp, c->st));
-  if (thb_out != thb_in) LK(launch_small_axpy(n, thb_in, thb_in, thb_in, 0.0, thb_out, c->st));
+  if (thb_out != thb_in)
+    PL(CL_SMALL, 0.0, launch_small_axpy(n, thb_in, thb_in, thb_in, 0.0, thb_out, c->st));
   const double* cur = th_in;
   for (int s = 0; s < ns; ++s) {
     RUN(gemm(c, n, c->Ng, o.T2S, cur, c->thsbp, epi_make(1.0)));
-    LK(launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->thsbp, o.pw, o.partner, c->partial,
+    PL(CL_PARSEVAL, 32.0 * n * (double)c->nm,
+       launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->thsbp, o.pw, o.partner, c->partial,
                        c->st));
-    LK(launch_sbp_substep(n, c->Pred, c->partial, o.d1b, o.thb2s, o.s2thb, mu * sdt, c->sbp,
+    PL(CL_SBP, 0.0,
+       launch_sbp_substep(n, c->Pred, c->partial, o.d1b, o.thb2s, o.s2thb, mu * sdt, c->sbp,
                           thb_out, c->g, c->st));
-    LK(launch_sbp_update(n, c->ld, c->thsbp, c->wsbp, c->g, sdt, c->st));
+    PL(CL_ELEMENTWISE, 48.0 * n * (double)c->nm,
+       launch_sbp_update(n, c->ld, c->thsbp, c->wsbp, c->g, sdt, c->st));
     RUN(gemm(c, c->Ng, n, o.S2T, c->thsbp, th_out, epi_make(1.0)));
     cur = th_out;
   }
@@ -245,9 +325,9 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
 
 int sanitize(nhqg_ctx* c) {
   const Ops& o = c->o;
-  LK(launch_sanitize(c->Nq, c->ld, c->P.n_pairs, o.pairs, 0, c->yq, c->st));
-  LK(launch_sanitize(c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yw, c->st));
-  LK(launch_sanitize(c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yth, c->st));
+  PL(CL_SMALL, 0.0, launch_sanitize(c->Nq, c->ld, c->P.n_pairs, o.pairs, 0, c->yq, c->st));
+  PL(CL_SMALL, 0.0, launch_sanitize(c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yw, c->st));
+  PL(CL_SMALL, 0.0, launch_sanitize(c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yth, c->st));
   if (c->P.apply_mask) {
     LK(launch_mask(c->Nq, c->ld, c->yq, o.mask, c->st));
     LK(launch_mask(c->Ng, c->ld, c->yw, o.mask, c->st));
@@ -284,7 +364,7 @@ int enqueue_step(nhqg_ctx* c) {
     RUN(combine(c, Ng, c->Rth, h));
   }
   RUN(implicit_solve(c, c->Rq, c->Rw, c->Rth, c->s1q, c->s1w, c->s1th));
-  LK(launch_small_matvec(Nq, Nq, o.proj_d, c->ythb, c->s1thb, c->st));
+  PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->ythb, c->s1thb, c->st));
   // stage 1 corrector
   const double* th1 = c->s1th;
   const double* thb1 = c->s1thb;
@@ -335,12 +415,12 @@ int enqueue_step(nhqg_ctx* c) {
     }
     RUN(combine(c, Ng, c->Rth, h));
     if (em)
-      LK(launch_small_axpy(Nq, c->ythb, thb1, c->s1thb, om / al, c->Rthb, c->st));
+      PL(CL_SMALL, 0.0, launch_small_axpy(Nq, c->ythb, thb1, c->s1thb, om / al, c->Rthb, c->st));
     else
-      LK(launch_small_axpy(Nq, c->ythb, c->ythb, c->ythb, 0.0, c->Rthb, c->st));
+      PL(CL_SMALL, 0.0, launch_small_axpy(Nq, c->ythb, c->ythb, c->ythb, 0.0, c->Rthb, c->st));
   }
   RUN(implicit_solve(c, c->Rq, c->Rw, c->Rth, c->yq, c->yw, c->yth));
-  LK(launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->ythb, c->st));
+  PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->ythb, c->st));
   if (em) RUN(corrector(c, c->yw, c->yth, c->ythb, c->yth, c->ythb, al));
   RUN(sanitize(c));
   return NHQG_OK;
@@ -506,12 +586,15 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(field(c, &c->tmpg, Ng));
   RUN(field(c, &c->wsbp, Nq));
   RUN(field(c, &c->thsbp, Nq));
+  RUN(field(c, &c->spq, Nq));
+  RUN(field(c, &c->spw, Ng));
+  RUN(field(c, &c->spth, Ng));
   RUN(dalloc(c, &c->partial, (size_t)Nq * c->Pred + 64));
-  RUN(dalloc(c, &c->sbp, (size_t)Nq * 16));
+  RUN(dalloc(c, &c->sbp, (size_t)Nq * 16 + 2 * (size_t)Nq * Nq));
   RUN(dalloc(c, &c->g, (size_t)Nq));
   c->fstride = (size_t)Nq * c->Nx * c->nky;
-  RUN(dalloc(c, &c->X5, c->fstride * 5));
-  RUN(dalloc(c, &c->Y6, c->fstride * 6));
+  RUN(dalloc(c, &c->X5, c->fstride * 8));
+  RUN(dalloc(c, &c->Y6, c->fstride * 9));
   RUN(dalloc(c, &c->staging, (size_t)Nq * c->Nx * c->Nk));
   RUN(dalloc(c, &c->diag, (size_t)64 + 8 * Nq));
   RUN(dalloc(c, &c->rowmax, (size_t)Nq * c->Nx));
@@ -844,15 +927,36 @@ int nhqg_get_aux(nhqg_ctx* c, double* q, double* w, double* th) {
 }
 
 int nhqg_profile_step(nhqg_ctx* c, double* out, int32_t n_out) {
-  (void)c;
-  (void)out;
-  (void)n_out;
-  return NHQG_ERR_UNSUPPORTED;
+  if (!c || !out) return NHQG_ERR_ARG;
+  if (!c->have_state) return NHQG_ERR_STATE;
+  if (n_out < 3 * CL_COUNT) {
+    c->err = "profile output needs 3 entries per kernel class";
+    return NHQG_ERR_ARG;
+  }
+  CK(hipStreamSynchronize(c->st));
+  c->recs.clear();
+  c->prof = true;
+  const int r = enqueue_step(c);
+  c->prof = false;
+  const hipError_t e = hipStreamSynchronize(c->st);
+  for (int i = 0; i < 3 * CL_COUNT; ++i) out[i] = 0.0;
+  for (auto& p : c->recs) {
+    float ms = 0.0f;
+    if (r == NHQG_OK && e == hipSuccess) (void)hipEventElapsedTime(&ms, p.a, p.b);
+    out[3 * p.cls] += ms;
+    out[3 * p.cls + 1] += 1.0;
+    out[3 * p.cls + 2] += p.work;
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  c->recs.clear();
+  if (r) return r;
+  CK(e);
+  return NHQG_OK;
 }
 
 const char* nhqg_timing_name(int32_t i) {
-  (void)i;
-  return "";
+  return (i >= 0 && i < CL_COUNT) ? kClassNames[i] : "";
 }
 
 int nhqg_device_bytes(nhqg_ctx* c, uint64_t* bytes) {

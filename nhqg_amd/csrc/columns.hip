@@ -122,20 +122,19 @@ hipError_t launch_parseval(int nl, int nm, int ld, int P, const double* a, const
 
 // ---------------------------------------------------------------------------
 // SBP2 corrector, serial 1-D part (nhqg/solver.py:929-942, 885-905).
-// Banded storage: band[i*5 + (j-i+2)] for |j-i| <= 2; d1b/lb hold the
-// tridiagonal SBP D1 and Laplacian as [i*3 + (j-i+1)].
-// sbp buffer layout (doubles, n = Nz+1):
-//   [0,5n) LU of A (multipliers below the diagonal), [5n,10n) B,
-//   [10n,11n) <w^2>, [11n,12n) flux, [12n,13n) thb_sbp, [13n,14n) rhs/new,
-//   [14n,15n) g
+// The corrector matrix A = I - (c1 L + c2 D1 diag(<w^2>) D1) with identity
+// boundary rows is pentadiagonal and fixed for all substeps of a call, so
+// the preparation kernel factors it in LDS (no pivoting: A is I plus a
+// positive semidefinite, diagonally dominant band) and forms A^{-1}
+// column by column (one column per thread, banded substitutions with the
+// recurrences in registers).  Each substep is then three small parallel
+// matrix-vector products.  d1b/lb hold the tridiagonal SBP D1 and
+// Laplacian as [i*3 + (j-i+1)].
+// sbp buffer (doubles, n = Nz+1): [0,5n) B band, [5n,10n) LU band,
+// [10n,11n) <w^2>, [11n,12n) forward-substitution scratch (n*n follows in
+// ainv), then ainv: n*n row-major A^{-1}, then ys: n*n scratch.
 // ---------------------------------------------------------------------------
-#define SBP_LU 0
-#define SBP_B 5
-#define SBP_W2 10
-#define SBP_FL 11
-#define SBP_TB 12
-#define SBP_NEW 13
-#define SBP_G 14
+#define SBP_NMAX 272
 
 NHQG_DEV double band3(const double* b3, int i, int j, int n) {
   const int o = j - i;
@@ -145,18 +144,17 @@ NHQG_DEV double band3(const double* b3, int i, int j, int n) {
 
 __global__ void __launch_bounds__(256)
 sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
-                   const double* __restrict__ lb, double c1, double c2, double* __restrict__ sbp) {
+                   const double* __restrict__ lb, double c1, double c2, double* __restrict__ Bg,
+                   double* __restrict__ ainv, double* __restrict__ ys) {
+  __shared__ double w2[SBP_NMAX];
+  __shared__ double LU[SBP_NMAX * 5];
   const int tid = threadIdx.x;
-  double* LU = sbp + SBP_LU * n;
-  double* B = sbp + SBP_B * n;
-  double* w2 = sbp + SBP_W2 * n;
   for (int i = tid; i < n; i += 256) {
     double s = 0.0;
     for (int p = 0; p < P; ++p) s += partial[(size_t)i * P + p];
     w2[i] = s;
   }
   __syncthreads();
-  // half = c1 * L + c2 * D1 diag(w2) D1 (pentadiagonal)
   for (int i = tid; i < n; i += 256) {
     for (int o = -2; o <= 2; ++o) {
       const int j = i + o;
@@ -169,43 +167,67 @@ sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const doubl
         }
       }
       const double id = (o == 0) ? 1.0 : 0.0;
-      B[i * 5 + o + 2] = id + h;
+      Bg[i * 5 + o + 2] = id + h;
       LU[i * 5 + o + 2] = (i == 0 || i == n - 1) ? id : id - h;
     }
   }
   __syncthreads();
-  // Banded LU without pivoting: A = I + (positive semidefinite, diagonally
-  // dominant band) with identity boundary rows, so elimination is stable.
   if (tid == 0) {
     for (int k = 0; k < n - 1; ++k) {
       const double piv = LU[k * 5 + 2];
       for (int i = k + 1; i <= min(k + 2, n - 1); ++i) {
         const double f = LU[i * 5 + (k - i) + 2] / piv;
         LU[i * 5 + (k - i) + 2] = f;
-        for (int j = k + 1; j <= min(k + 2, n - 1); ++j) LU[i * 5 + (j - i) + 2] -= f * LU[k * 5 + (j - k) + 2];
+        for (int j = k + 1; j <= min(k + 2, n - 1); ++j)
+          LU[i * 5 + (j - i) + 2] -= f * LU[k * 5 + (j - k) + 2];
       }
+    }
+  }
+  __syncthreads();
+  // column c of A^{-1}: L y = e_c, U x = y; ys/ainv rows indexed by c
+  for (int c = tid; c < n; c += 256) {
+    double* y = ys + (size_t)c * n;
+    double y1 = 0.0, y2 = 0.0;  // y[i-1], y[i-2]
+    for (int i = 0; i < n; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      if (i >= 1) v -= LU[i * 5 + 1] * y1;
+      if (i >= 2) v -= LU[i * 5 + 0] * y2;
+      y[i] = v;
+      y2 = y1;
+      y1 = v;
+    }
+    double x1 = 0.0, x2 = 0.0;  // x[i+1], x[i+2]
+    for (int i = n - 1; i >= 0; --i) {
+      double v = y[i];
+      if (i + 1 < n) v -= LU[i * 5 + 3] * x1;
+      if (i + 2 < n) v -= LU[i * 5 + 4] * x2;
+      v /= LU[i * 5 + 2];
+      ainv[(size_t)i * n + c] = v;
+      x2 = x1;
+      x1 = v;
     }
   }
 }
 
-// One substep's 1-D solve: flux -> rhs -> thb_sbp_new -> g, thb.
+// One substep's 1-D solve: flux -> rhs -> thb_sbp_new = A^{-1} rhs -> g, thb.
 __global__ void __launch_bounds__(256)
 sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
                    const double* __restrict__ thb2s, const double* __restrict__ s2thb,
-                   double mu_sdt, double* __restrict__ sbp, double* __restrict__ thb_io,
-                   double* __restrict__ g_out) {
+                   double mu_sdt, const double* __restrict__ Bg, const double* __restrict__ ainv,
+                   double* __restrict__ thb_io, double* __restrict__ g_out) {
+  __shared__ double fl[SBP_NMAX], tb[SBP_NMAX], rh[SBP_NMAX], nw[SBP_NMAX], th0[SBP_NMAX];
   const int tid = threadIdx.x;
-  const double* LU = sbp + SBP_LU * n;
-  const double* B = sbp + SBP_B * n;
-  double* fl = sbp + SBP_FL * n;
-  double* tb = sbp + SBP_TB * n;
-  double* nw = sbp + SBP_NEW * n;
+  for (int i = tid; i < n; i += 256) th0[i] = thb_io[i];
   for (int i = tid; i < n; i += 256) {
     double s = 0.0;
     for (int p = 0; p < P; ++p) s += partial[(size_t)i * P + p];
     fl[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
     double t = 0.0;
-    for (int j = 0; j < n; ++j) t += thb2s[(size_t)i * n + j] * thb_io[j];
+    const double* r = thb2s + (size_t)i * n;
+    for (int j = 0; j < n; ++j) t += r[j] * th0[j];
     tb[i] = t;
   }
   __syncthreads();
@@ -213,31 +235,31 @@ sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const doubl
     double r = 0.0;
     for (int o = -2; o <= 2; ++o) {
       const int j = i + o;
-      if (j >= 0 && j < n) r += B[i * 5 + o + 2] * tb[j];
+      if (j >= 0 && j < n) r += Bg[i * 5 + o + 2] * tb[j];
     }
     double d = 0.0;
-    for (int o = -1; o <= 1; ++o) d += band3(d1b, i, i + o, n) * ((i + o >= 0 && i + o < n) ? fl[i + o] : 0.0);
-    r -= mu_sdt * d;
-    nw[i] = (i == 0 || i == n - 1) ? 0.0 : r;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    for (int i = 1; i < n; ++i)
-      for (int k = max(0, i - 2); k < i; ++k) nw[i] -= LU[i * 5 + (k - i) + 2] * nw[k];
-    for (int i = n - 1; i >= 0; --i) {
-      double v = nw[i];
-      for (int j = i + 1; j <= min(i + 2, n - 1); ++j) v -= LU[i * 5 + (j - i) + 2] * nw[j];
-      nw[i] = v / LU[i * 5 + 2];
+    for (int o = -1; o <= 1; ++o) {
+      const int j = i + o;
+      if (j >= 0 && j < n) d += band3(d1b, i, j, n) * fl[j];
     }
+    r -= mu_sdt * d;
+    rh[i] = (i == 0 || i == n - 1) ? 0.0 : r;
   }
   __syncthreads();
   for (int i = tid; i < n; i += 256) {
-    double gsum = 0.0;
+    double t = 0.0;
+    const double* r = ainv + (size_t)i * n;
+    for (int j = 0; j < n; ++j) t += r[j] * rh[j];
+    nw[i] = t;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
+    double gs = 0.0;
     for (int o = -1; o <= 1; ++o) {
       const int j = i + o;
-      if (j >= 0 && j < n) gsum += band3(d1b, i, j, n) * (tb[j] + nw[j]);
+      if (j >= 0 && j < n) gs += band3(d1b, i, j, n) * (tb[j] + nw[j]);
     }
-    g_out[i] = 0.5 * gsum;
+    g_out[i] = 0.5 * gs;
   }
   __syncthreads();
   if (tid == 0) {
@@ -247,7 +269,8 @@ sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const doubl
   __syncthreads();
   for (int i = tid; i < n; i += 256) {
     double t = 0.0;
-    for (int j = 0; j < n; ++j) t += s2thb[(size_t)i * n + j] * nw[j];
+    const double* r = s2thb + (size_t)i * n;
+    for (int j = 0; j < n; ++j) t += r[j] * nw[j];
     thb_io[i] = t;
   }
 }
@@ -255,16 +278,23 @@ sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const doubl
 hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double* d1b,
                               const double* lb, double c1, double c2, double* sbp,
                               hipStream_t s) {
+  if (n > SBP_NMAX) return hipErrorInvalidValue;
+  double* Bg = sbp;
+  double* ainv = sbp + 16 * (size_t)n;
+  double* ys = ainv + (size_t)n * n;
   hipLaunchKernelGGL(sbp_prepare_kernel, dim3(1), dim3(256), 0, s, n, P, partial, d1b, lb, c1, c2,
-                     sbp);
+                     Bg, ainv, ys);
   return hipGetLastError();
 }
 
 hipError_t launch_sbp_substep(int n, int P, const double* partial, const double* d1b,
                               const double* thb2s, const double* s2thb, double mu_sdt,
                               double* sbp, double* thb_io, double* g_out, hipStream_t s) {
+  if (n > SBP_NMAX) return hipErrorInvalidValue;
+  const double* Bg = sbp;
+  const double* ainv = sbp + 16 * (size_t)n;
   hipLaunchKernelGGL(sbp_substep_kernel, dim3(1), dim3(256), 0, s, n, P, partial, d1b, thb2s,
-                     s2thb, mu_sdt, sbp, thb_io, g_out);
+                     s2thb, mu_sdt, Bg, ainv, thb_io, g_out);
   return hipGetLastError();
 }
 

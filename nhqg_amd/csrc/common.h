@@ -78,16 +78,19 @@ static inline void epi_add(Epi& e, double a, const double* X, const double* s = 
 
 // ---- launch wrappers (defined in the .hip files) ----
 struct FluxArgs {
-  int N, nq, nkx, nky, ldc;
-  const double2 *qn, *wn, *thn;
-  double2 *aq, *aw, *ath;
+  int N, nq, nkx, nky, ldc, ng;
+  const double2 *qn, *wn, *thn;   // nodal fields (advection)
+  const double2 *qc, *wc, *thc;   // coefficient fields (sponge)
+  double2 *aq, *aw, *ath;          // advection outputs (nodal spectral)
+  double2 *sq, *sw, *sth;          // sponge outputs (coefficient levels)
   const int *kidx_of_x, *rows;
   const double *kx_row, *ky_col, *inv_denom, *mask, *eta_phys, *sponge_phys;
   const double2* tw;
   double2 *X5, *Y6;
   size_t fstride;
 };
-hipError_t launch_flux(const FluxArgs& a, hipStream_t s);
+// pass: 0 all three passes, 1 col_inverse, 2 row products, 3 col_forward
+hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s);
 hipError_t launch_speed(const FluxArgs& a, double* rowmax, hipStream_t s);
 hipError_t launch_shell_solve(int n_shells, int Ng, int ld, const int* shell_ptr,
                               const int* shell_modes, const double* invT, const double* rhs,

@@ -11,17 +11,21 @@
 //   col_inverse : per (level, ky): inverse DFT along x of
 //                 U = i ky psi, V = -i kx psi, Q, W, Theta (psi = -q / |k|^2
 //                 is formed on the fly from the nodal q)           -> X5
-//   row_flux    : per (level, x):  packed c2r along y (two real rows per
+//   row_flux    : per (level, x):  c2r along y (u and v packed in one
 //                 complex FFT), pointwise products u(q+eta), v(q+eta), uw,
 //                 vw, u theta, v theta, sigma q, sigma w, sigma theta,
-//                 packed r2c along y; applies i ky and the sponge    -> Y6
+//                 r2c along y ((u f, v f) packed); applies i ky and the
+//                 sponge                                           -> Y6
 //   col_forward : per (level, ky): forward DFT along x, i kx G + H,
 //                 gather of the kept rows (2/3 band)                -> A
 //
 // A "team" of Nx/4 threads owns one transform; a 256-thread block holds
-// 1024/Nx teams.  Sponge and trap are folded into the same passes: the
-// sponge acts on nodal fields, which equals acting on coefficient levels
-// because sigma is independent of Z and the vertical maps are linear.
+// 1024/Nx teams.  The trap background is added to q' in physical space.
+// The sponge rides in the same passes but, as in the reference, acts on
+// the coefficient levels of q', w and theta (X5 slots 5-7, Y6 slots 6-8):
+// acting on nodal values instead is equal in exact arithmetic, but the
+// transform back to coefficients then leaves roundoff of the large nodal
+// products in the tiny high Chebyshev coefficients, which d/dZ amplifies.
 #include "common.h"
 
 template <int N> struct Log2;
@@ -104,14 +108,17 @@ NHQG_DEV void load_twiddles(double2* tw, const double2* twg) {
 // qn/wn/thn: nodal spectral fields (level-major, row stride ldc double2).
 // X5[f][l][x][ky], f = U, V, Q, W, Theta.
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, bool SPONGE>
 __global__ void __launch_bounds__(256)
-col_inverse_flux(int nkx, int nky, int ldc, const double2* __restrict__ qn,
+col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ qn,
                  const double2* __restrict__ wn, const double2* __restrict__ thn,
-                 const int* __restrict__ kidx_of_x, const double* __restrict__ kx_row,
-                 const double* __restrict__ ky_col, const double* __restrict__ inv_denom,
-                 const double2* __restrict__ twg, double2* __restrict__ X5, size_t fstride) {
+                 const double2* __restrict__ qc, const double2* __restrict__ wc,
+                 const double2* __restrict__ thc, const int* __restrict__ kidx_of_x,
+                 const double* __restrict__ kx_row, const double* __restrict__ ky_col,
+                 const double* __restrict__ inv_denom, const double2* __restrict__ twg,
+                 double2* __restrict__ X5, size_t fstride) {
   constexpr int T = N / 4, TEAMS = 256 / T;
+  constexpr int NF = SPONGE ? 8 : 5;
   __shared__ double2 buf[2][TEAMS][N];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
@@ -121,7 +128,9 @@ col_inverse_flux(int nkx, int nky, int ldc, const double2* __restrict__ qn,
   const bool active = ky < nky;
   load_twiddles<N>(tw, twg);
   const size_t lb = (size_t)l * ldc;
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < NF; ++f) {
+    // w and theta coefficient fields have ng < Nz+1 levels (block-uniform)
+    if (f >= 6 && l >= ng) continue;
     for (int x = t; x < N; x += T) {
       double2 v = make_double2(0.0, 0.0);
       const int i = kidx_of_x[x];
@@ -132,8 +141,10 @@ col_inverse_flux(int nkx, int nky, int ldc, const double2* __restrict__ qn,
           if (f == 0) v = cmul_i(qv, ky_col[ky] * inv_denom[m]);
           else if (f == 1) v = cmul_i(qv, -kx_row[i] * inv_denom[m]);
           else v = qv;
-        } else {
+        } else if (f < 5) {
           v = (f == 3 ? wn : thn)[lb + m];
+        } else {
+          v = (f == 5 ? qc : f == 6 ? wc : thc)[lb + m];
         }
       }
       buf[0][team][x] = v;
@@ -221,7 +232,7 @@ NHQG_DEV void inv_pair(double2* ba, double2* bb, int t, const double2* tw, int n
 // ---------------------------------------------------------------------------
 template <int N, bool SPONGE, bool TRAP>
 __global__ void __launch_bounds__(256)
-row_flux(int nky, const double2* __restrict__ X5, size_t fstride, double2* __restrict__ Y6,
+row_flux(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double2* __restrict__ Y6,
          const double* __restrict__ ky_col, const double* __restrict__ eta_phys,
          const double* __restrict__ sponge_phys, const double2* __restrict__ twg) {
   constexpr int T = N / 4, TEAMS = 256 / T;
@@ -238,33 +249,20 @@ row_flux(int nky, const double2* __restrict__ X5, size_t fstride, double2* __res
   double2* bb = buf[1][team];
   const size_t ro = ((size_t)l * N + x) * nky;
   const double sc = 1.0 / ((double)N * (double)N);
+  // Only rows that end in the same output field share a packed FFT: the
+  // unpacking leaks roundoff of one member into the other, which is
+  // harmless at equal scale (u with v; u f with v f) but not across fields
+  // of very different size (a strong barotropic q next to a weak w).
   double u[4], v[4], q[4], w[4], th[4];
   inv_pair<N>(ba, bb, t, tw, nky, X5 + 0 * fstride + ro, X5 + 1 * fstride + ro, u, v, sc);
-  inv_pair<N>(ba, bb, t, tw, nky, X5 + 2 * fstride + ro, X5 + 3 * fstride + ro, q, w, sc);
+  inv_pair<N>(ba, bb, t, tw, nky, X5 + 2 * fstride + ro, nullptr, q, nullptr, sc);
+  inv_pair<N>(ba, bb, t, tw, nky, X5 + 3 * fstride + ro, nullptr, w, nullptr, sc);
   inv_pair<N>(ba, bb, t, tw, nky, X5 + 4 * fstride + ro, nullptr, th, nullptr, sc);
-
-  double2 Sq[3], Sw[3], St[3];
-  for (int s = 0; s < 3; ++s) Sq[s] = Sw[s] = St[s] = make_double2(0.0, 0.0);
-  if (SPONGE) {
-    double a[4], b[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const double sg = sponge_phys[(size_t)x * N + t + j * T];
-      a[j] = sg * q[j];
-      b[j] = sg * w[j];
-    }
-    fwd_pair<N>(ba, bb, t, tw, a, b, nky, Sq, Sw);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) a[j] = sponge_phys[(size_t)x * N + t + j * T] * th[j];
-    double2 dummy[3];
-    fwd_pair<N>(ba, bb, t, tw, a, nullptr, nky, St, dummy);
-  }
   if (TRAP) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) q[j] += eta_phys[(size_t)x * N + t + j * T];
   }
-  const double* fld[3] = {q, w, th};
-  const double2* S[3] = {Sq, Sw, St};
+  double* fld[3] = {q, w, th};
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     double a[4], b[4];
@@ -283,7 +281,28 @@ row_flux(int nky, const double2* __restrict__ X5, size_t fstride, double2* __res
         const int ky = t + s * T;
         if (ky < nky) {
           dg[ky] = G[s];
-          dh[ky] = cadd(cmul_i(Hv[s], ky_col[ky]), S[f][s]);
+          dh[ky] = cmul_i(Hv[s], ky_col[ky]);
+        }
+      }
+    }
+  }
+  if (SPONGE) {
+    // sigma times each coefficient level of q', w, theta (spectral.py:53-60)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (f > 0 && l >= ng) continue;  // block-uniform
+      double c[4], a[4];
+      inv_pair<N>(ba, bb, t, tw, nky, X5 + (5 + f) * fstride + ro, nullptr, c, nullptr, sc);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[j] = sponge_phys[(size_t)x * N + t + j * T] * c[j];
+      double2 S[3], dummy[3];
+      fwd_pair<N>(ba, bb, t, tw, a, nullptr, nky, S, dummy);
+      if (active) {
+        double2* ds = Y6 + (6 + f) * fstride + ro;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const int ky = t + s * T;
+          if (ky < nky) ds[ky] = S[s];
         }
       }
     }
@@ -294,13 +313,14 @@ row_flux(int nky, const double2* __restrict__ X5, size_t fstride, double2* __res
 // Pass 3: forward DFT along x; A_f(kx, ky) = i kx F_x(G_f) + F_x(H_f) at the
 // kept rows, times the output mask.  Writes A into the nodal buffers.
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, bool SPONGE>
 __global__ void __launch_bounds__(256)
-col_forward_flux(int nkx, int nky, int ldc, const double2* __restrict__ Y6, size_t fstride,
-                 const int* __restrict__ rows, const double* __restrict__ kx_row,
+col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ Y6,
+                 size_t fstride, const int* __restrict__ rows, const double* __restrict__ kx_row,
                  const double* __restrict__ mask, double2* __restrict__ out_q,
                  double2* __restrict__ out_w, double2* __restrict__ out_th,
-                 const double2* __restrict__ twg) {
+                 double2* __restrict__ sp_q, double2* __restrict__ sp_w,
+                 double2* __restrict__ sp_th, const double2* __restrict__ twg) {
   constexpr int T = N / 4, TEAMS = 256 / T;
   __shared__ double2 buf[2][TEAMS][N];
   __shared__ double2 tw[N];
@@ -311,12 +331,16 @@ col_forward_flux(int nkx, int nky, int ldc, const double2* __restrict__ Y6, size
   const bool active = ky < nky;
   load_twiddles<N>(tw, twg);
   double2* outs[3] = {out_q, out_w, out_th};
+  double2* sps[3] = {sp_q, sp_w, sp_th};
+  constexpr int NG = SPONGE ? 3 : 2;  // G, H and (sponge) S per field
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     double2 g[4];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const double2* src = Y6 + (2 * f + h) * fstride + (size_t)l * N * nky;
+    for (int h = 0; h < NG; ++h) {
+      if (h == 2 && f > 0 && l >= ng) continue;  // block-uniform
+      const int slot = (h < 2) ? 2 * f + h : 6 + f;
+      const double2* src = Y6 + slot * fstride + (size_t)l * N * nky;
       for (int idx = tid; idx < N * TEAMS; idx += 256) {
         const int cc = idx % TEAMS, x = idx / TEAMS;
         const int kyc = ky0 + cc;
@@ -329,11 +353,13 @@ col_forward_flux(int nkx, int nky, int ldc, const double2* __restrict__ Y6, size
         const int i = t + j * T;
         if (i < nkx) {
           const double2 z = r[rows[i]];
+          const int m = ky * nkx + i;
           if (h == 0) {
             g[j] = cmul_i(z, kx_row[i]);
+          } else if (h == 1) {
+            if (active) outs[f][(size_t)l * ldc + m] = cscale(cadd(g[j], z), mask[m]);
           } else if (active) {
-            const int m = ky * nkx + i;
-            outs[f][(size_t)l * ldc + m] = cscale(cadd(g[j], z), mask[m]);
+            sps[f][(size_t)l * ldc + m] = cscale(z, mask[m]);
           }
         }
       }
@@ -383,40 +409,45 @@ row_speed(int nky, const double2* __restrict__ X5, size_t fstride, double* __res
 // ---------------------------------------------------------------------------
 
 
-template <int N>
-static hipError_t flux_n(const FluxArgs& a, hipStream_t s) {
+template <int N, bool SP>
+static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;
   dim3 gc((a.nky + TEAMS - 1) / TEAMS, a.nq);
-  hipLaunchKernelGGL(col_inverse_flux<N>, gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.qn, a.wn,
-                     a.thn, a.kidx_of_x, a.kx_row, a.ky_col, a.inv_denom, a.tw, a.X5, a.fstride);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
-  const bool sp = a.sponge_phys != nullptr, tr = a.eta_phys != nullptr;
-  if (sp && tr)
-    hipLaunchKernelGGL((row_flux<N, true, true>), gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
-                       a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
-  else if (sp)
-    hipLaunchKernelGGL((row_flux<N, true, false>), gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
-                       a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
-  else if (tr)
-    hipLaunchKernelGGL((row_flux<N, false, true>), gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
-                       a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
-  else
-    hipLaunchKernelGGL((row_flux<N, false, false>), gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
-                       a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
-  hipLaunchKernelGGL(col_forward_flux<N>, gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.Y6,
-                     a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath, a.tw);
+  if (pass == 0 || pass == 1)
+    hipLaunchKernelGGL((col_inverse_flux<N, SP>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.ng,
+                       a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row, a.ky_col,
+                       a.inv_denom, a.tw, a.X5, a.fstride);
+  if (pass == 0 || pass == 2) {
+    if (a.eta_phys)
+      hipLaunchKernelGGL((row_flux<N, SP, true>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
+                         a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+    else
+      hipLaunchKernelGGL((row_flux<N, SP, false>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
+                         a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+  }
+  if (pass == 0 || pass == 3)
+    hipLaunchKernelGGL((col_forward_flux<N, SP>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.ng,
+                       a.Y6, a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath, a.sq, a.sw,
+                       a.sth, a.tw);
+}
+
+template <int N>
+static hipError_t flux_n(const FluxArgs& a, int pass, hipStream_t s) {
+  if (a.sponge_phys) flux_pass<N, true>(a, pass, s);
+  else flux_pass<N, false>(a, pass, s);
   return hipGetLastError();
 }
 
-hipError_t launch_flux(const FluxArgs& a, hipStream_t s) {
+hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s) {
   switch (a.N) {
-    case 16: return flux_n<16>(a, s);
-    case 32: return flux_n<32>(a, s);
-    case 64: return flux_n<64>(a, s);
-    case 128: return flux_n<128>(a, s);
-    case 256: return flux_n<256>(a, s);
-    case 512: return flux_n<512>(a, s);
-    case 1024: return flux_n<1024>(a, s);
+    case 16: return flux_n<16>(a, pass, s);
+    case 32: return flux_n<32>(a, pass, s);
+    case 64: return flux_n<64>(a, pass, s);
+    case 128: return flux_n<128>(a, pass, s);
+    case 256: return flux_n<256>(a, pass, s);
+    case 512: return flux_n<512>(a, pass, s);
+    case 1024: return flux_n<1024>(a, pass, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -425,8 +456,9 @@ template <int N>
 static hipError_t speed_n(const FluxArgs& a, double* rowmax, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;
   dim3 gc((a.nky + TEAMS - 1) / TEAMS, a.nq);
-  hipLaunchKernelGGL(col_inverse_flux<N>, gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.qn, a.qn,
-                     a.qn, a.kidx_of_x, a.kx_row, a.ky_col, a.inv_denom, a.tw, a.X5, a.fstride);
+  hipLaunchKernelGGL((col_inverse_flux<N, false>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, 0,
+                     a.qn, a.qn, a.qn, a.qn, a.qn, a.qn, a.kidx_of_x, a.kx_row, a.ky_col,
+                     a.inv_denom, a.tw, a.X5, a.fstride);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
   hipLaunchKernelGGL(row_speed<N>, gr, dim3(256), 0, s, a.nky, a.X5, a.fstride, rowmax, a.tw);
   return hipGetLastError();

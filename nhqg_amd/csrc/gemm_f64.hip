@@ -11,59 +11,110 @@
 // that fuses the stage-assembly arithmetic.
 #include "common.h"
 
-#define KC 16   // k-chunk staged in LDS per iteration
+#define KC 8    // k-chunk staged in LDS per iteration (double buffered)
 
+// Wave w owns row tiles w, w+4, w+8, ... and all four 16-column tiles of
+// the block, so each A value read from LDS feeds 4 MFMAs and each B value
+// feeds up to MTW.  LDS holds k-major chunks (rows contiguous); the next
+// chunk is fetched into registers while the current one is multiplied.
 template <int MT>
 __global__ void __launch_bounds__(256)
 gemm_f64_kernel(int M, int K, const double* __restrict__ A, int lda,
                 const double* __restrict__ B, double* C, int ld, Epi e) {
-  __shared__ double As[MT * 16][KC + 1];
-  __shared__ double Bs[KC][64 + 1];
+  constexpr int MP = MT * 16;
+  constexpr int SA = MP + 2;          // padded row length of the A chunk
+  constexpr int SB = 64 + 2;
+  constexpr int MTW = (MT + 3) / 4;   // row tiles per wave
+  constexpr int NA = (MP * KC + 255) / 256;
+  constexpr int NB = (KC * 64) / 256;
+  __shared__ double As[2][KC][SA];
+  __shared__ double Bs[2][KC][SB];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const int n0 = blockIdx.x * 64;
-  const int li = lane & 15;   // row of A / column of B and D held by this lane
-  const int lk = lane >> 4;   // k index (A, B) and row quad (D)
+  const int li = lane & 15;
+  const int lk = lane >> 4;
 
-  v4d acc[MT];
+  v4d acc[MTW][4];
 #pragma unroll
-  for (int t = 0; t < MT; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < MTW; ++i)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[i][c] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  for (int k0 = 0; k0 < K; k0 += KC) {
-    for (int idx = tid; idx < MT * 16 * KC; idx += 256) {
-      const int r = idx / KC, c = idx % KC;
-      As[r][c] = (r < M && k0 + c < K) ? A[(size_t)r * lda + k0 + c] : 0.0;
+  double ra[NA], rb[NB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int idx = tid + u * 256;        // idx = r * KC + kc (kc fastest)
+      const int r = idx / KC, kc = idx % KC;
+      ra[u] = (idx < MP * KC && r < M && k0 + kc < K) ? A[(size_t)r * lda + k0 + kc] : 0.0;
     }
-    for (int idx = tid; idx < KC * 64; idx += 256) {
-      const int r = idx >> 6, c = idx & 63;
-      Bs[r][c] = (k0 + r < K) ? B[(size_t)(k0 + r) * ld + n0 + c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int idx = tid + u * 256;        // idx = kr * 64 + c
+      const int kr = idx >> 6, cc = idx & 63;
+      rb[u] = (k0 + kr < K) ? B[(size_t)(k0 + kr) * ld + n0 + cc] : 0.0;
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int idx = tid + u * 256;
+      if (idx < MP * KC) As[buf][idx % KC][idx / KC] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int idx = tid + u * 256;
+      Bs[buf][idx >> 6][idx & 63] = rb[u];
+    }
+  };
+
+  const int nchunks = (K + KC - 1) / KC;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int cur = ch & 1;
+    if (ch + 1 < nchunks) fetch((ch + 1) * KC);
 #pragma unroll
     for (int kk = 0; kk < KC / 4; ++kk) {
-      const double b = Bs[kk * 4 + lk][wave * 16 + li];
+      const int kr = kk * 4 + lk;
+      double b[4];
 #pragma unroll
-      for (int t = 0; t < MT; ++t) {
-        const double a = As[t * 16 + li][kk * 4 + lk];
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+      for (int c = 0; c < 4; ++c) b[c] = Bs[cur][kr][c * 16 + li];
+#pragma unroll
+      for (int i = 0; i < MTW; ++i) {
+        const int t = wave + 4 * i;
+        if (t < MT) {
+          const double a = As[cur][kr][t * 16 + li];
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[c], acc[i][c], 0, 0, 0);
+        }
       }
     }
+    if (ch + 1 < nchunks) stash(cur ^ 1);
     __syncthreads();
   }
 
-  // v_mfma_f64_16x16x4_f64 output: D[i][j] of tile t sits in lane
+  // v_mfma_f64_16x16x4_f64 output: D[i][j] of a tile sits in lane
   // j + 16*(i%4), register i/4 (one row group per register).
-  const int col = n0 + wave * 16 + li;
-  const int mode = col >> 1;
 #pragma unroll
-  for (int t = 0; t < MT; ++t) {
+  for (int i = 0; i < MTW; ++i) {
+    const int t = wave + 4 * i;
+    if (t >= MT) continue;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = t * 16 + r * 4 + lk;
-      if (row < M) {
-        const size_t idx = (size_t)row * ld + col;
-        C[idx] = epi_apply(e, acc[t][r], idx, mode);
+    for (int c = 0; c < 4; ++c) {
+      const int col = n0 + c * 16 + li;
+      const int mode = col >> 1;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = t * 16 + r * 4 + lk;
+        if (row < M) {
+          const size_t idx = (size_t)row * ld + col;
+          C[idx] = epi_apply(e, acc[i][c][r], idx, mode);
+        }
       }
     }
   }

@@ -40,8 +40,12 @@ def thermal_noise_state(grid: Grid, seed: int = 0, amplitude: float = 1e-3,
     pat = np.fft.irfft2(nh * filt * grid.mask_23, s=(Nx, Nx))
     pat *= amplitude / pat.std()
     ph = np.fft.rfft2(pat) * grid.mask_23
-    nodal = np.sin(np.pi * grid.Z)[:, None, None] * ph[None]
-    cheb = np.tensordot(grid.proj_dirichlet @ grid.V_inv, nodal, axes=1)
+    # Chebyshev coefficients of the sin(pi Z) envelope, with the ones far
+    # below roundoff (they decay like (pi/2)^n / n! and reach the subnormal
+    # range at Nz >= 128) set to zero
+    env = grid.proj_dirichlet @ (grid.V_inv @ np.sin(np.pi * grid.Z))
+    env[np.abs(env) < 1e-30 * np.abs(env).max()] = 0.0
+    cheb = env[:, None, None] * ph[None]
     th = np.tensordot(grid.dirichlet_pinv, cheb, axes=1)
     Nq, Ng = grid.Nz + 1, grid.Nz - 1
     q = np.zeros((Nq, Nx, grid.Nk), np.complex128)

@@ -113,7 +113,7 @@ def linear_interp_matrix(src: np.ndarray, dst: np.ndarray) -> np.ndarray:
             t = (x - s[j]) / (s[j + 1] - s[j])
             M[i, j] = 1.0 - t
             M[i, j + 1] = t
-    return M[:, ::-1] if desc else M
+    return np.ascontiguousarray(M[:, ::-1]) if desc else M
 
 
 def sbp2(N: int):
@@ -333,6 +333,7 @@ class OState:
 
 def vmat(M: np.ndarray, f: np.ndarray) -> np.ndarray:
     """Apply a real vertical matrix to a level-major field (einsum 'ij,j...')."""
+    M = np.ascontiguousarray(M)
     if np.iscomplexobj(f):
         fr = np.ascontiguousarray(f).view(np.float64)
         out = (M @ fr.reshape(fr.shape[0], -1)).reshape((M.shape[0],) + fr.shape[1:])

@@ -51,3 +51,21 @@ def active_state(cfg, seed=0, amp=1e-2):
     thb = o.proj_d @ thb
     st = O.sanitize(O.OState(s.q, w, th, thb), o)
     return st
+
+
+def oracle_reordered(fn, *args):
+    """Run an oracle function with every vertical contraction summed in a
+    different (equally valid) order: the spread against the plain oracle is
+    the roundoff noise floor of the computation."""
+    orig = O.vmat
+
+    def split(M, f):
+        M = np.ascontiguousarray(M)
+        h = M.shape[1] // 2 + 1
+        return orig(np.ascontiguousarray(M[:, h:]), f[h:]) + orig(np.ascontiguousarray(M[:, :h]), f[:h])
+
+    O.vmat = split
+    try:
+        return fn(*args)
+    finally:
+        O.vmat = orig

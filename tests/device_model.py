@@ -85,14 +85,12 @@ class DeviceModel:
             return A, B
 
         u, v = inv_pair(X[0], X[1])
-        q, w = inv_pair(X[2], X[3])
+        q, _ = inv_pair(X[2], None)
+        w, _ = inv_pair(X[3], None)
         th, _ = inv_pair(X[4], None)
         sg = o["sponge_phys"]
         eta = o["eta_phys"]
         S = [np.zeros((nl, N, nky), complex) for _ in range(3)]
-        if sg is not None:
-            S[0], S[1] = fwd_pair(sg * q, sg * w)
-            S[2], _ = fwd_pair(sg * th, 0 * th)
         qa = q + (eta if eta is not None else 0.0)
         out = []
         for f, Sf in ((qa, S[0]), (w, S[1]), (th, S[2])):
@@ -106,6 +104,11 @@ class DeviceModel:
         return out
 
     # ---- step pieces ----
+    def sponge(self, c):
+        g = self.g
+        f = np.fft.irfft2(self.to_full(c, c.shape[0]), s=(g.Nx, g.Nx))
+        return self.to_compact(np.fft.rfft2(self.o["sponge_phys"][None] * f)) * self.t["mask"]
+
     def explicit_rhs(self, q, w, th):
         o = self.o
         qn, wn, thn = o["V"] @ q, o["VSw"] @ w, o["VSd"] @ th
@@ -113,7 +116,10 @@ class DeviceModel:
         Eq = -(o["Vinv"] @ Aq)
         if self.g.beta != 0.0:
             Eq = Eq + 1j * self.g.beta * self.t["kx_mode"] * self.t["inv_denom"] * q
-        return Eq, -(o["Pw"] @ Aw), -(o["Pd"] @ Ath)
+        Ew, Eth = -(o["Pw"] @ Aw), -(o["Pd"] @ Ath)
+        if o["sponge_phys"] is not None:
+            Eq, Ew, Eth = Eq - self.sponge(q), Ew - self.sponge(w), Eth - self.sponge(th)
+        return Eq, Ew, Eth
 
     def shell_apply(self, rhs):
         t, o = self.t, self.o
