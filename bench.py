@@ -139,6 +139,9 @@ def main() -> int:
     dev_ms = sp.last_ms()
     diag = sp.diagnostics()
     prof = sp.ctx.profile_step()
+    # plain 1 GiB device copy: practical HBM ceiling and the known-size
+    # dispatch that calibrates the FETCH_SIZE / WRITE_SIZE counters
+    copy_gbps = sp.ctx.stream_copy(1 << 30, reps=5)
 
     res = {
         "metric": METRIC,
@@ -162,6 +165,7 @@ def main() -> int:
             "parallelism": "1 GPU",
         },
         "roofline": roofline(prof, g, lay),
+        "hbm_copy_gbps": copy_gbps,
         "kernels": {k: {"ms_per_step": v[0], "launches": v[1]} for k, v in prof.items()},
         "diagnostics_after": {k: diag[k] for k in ("KE_tot", "max_speed", "Nusselt_dealiased")},
     }

@@ -169,6 +169,12 @@ int nhqg_last_step_ms(nhqg_ctx *ctx, double *ms);
 int nhqg_profile_step(nhqg_ctx *ctx, double *out, int32_t n_out);
 const char *nhqg_timing_name(int32_t i);
 
+/* Measurement utility with no counterpart in the reference: a plain
+ * streaming copy of `bytes` (read + write), repeated `reps` times on the
+ * context stream; returns the achieved GB/s (read + write bytes).  Used to
+ * calibrate HBM traffic counters and as a practical bandwidth ceiling. */
+int nhqg_stream_copy(nhqg_ctx *ctx, uint64_t bytes, int32_t reps, double *gbps);
+
 /* Bytes of device memory held by the context. */
 int nhqg_device_bytes(nhqg_ctx *ctx, uint64_t *bytes);
 

@@ -59,7 +59,7 @@ EXPORTS = (
     "nhqg_create", "nhqg_destroy", "nhqg_set_state", "nhqg_get_state", "nhqg_step",
     "nhqg_step_async", "nhqg_sync", "nhqg_diagnostics", "nhqg_apply", "nhqg_get_aux",
     "nhqg_last_step_ms", "nhqg_profile_step", "nhqg_timing_name", "nhqg_device_bytes",
-    "nhqg_last_error",
+    "nhqg_last_error", "nhqg_stream_copy",
 )
 
 
@@ -96,6 +96,7 @@ def load():
     lib.nhqg_timing_name.argtypes = [C.c_int32]
     lib.nhqg_timing_name.restype = C.c_char_p
     lib.nhqg_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+    lib.nhqg_stream_copy.argtypes = [vp, C.c_uint64, C.c_int32, _dp]
     lib.nhqg_last_error.argtypes = [vp]
     lib.nhqg_last_error.restype = C.c_char_p
     _lib = lib
@@ -246,6 +247,13 @@ class Context:
             if name:
                 res[name] = (float(out[3 * i]), int(out[3 * i + 1]), float(out[3 * i + 2]))
         return res
+
+    def stream_copy(self, nbytes, reps=10):
+        """Achieved GB/s (read + write) of a plain device copy of nbytes."""
+        v = C.c_double()
+        self._check(self.lib.nhqg_stream_copy(self.h, int(nbytes), int(reps), C.byref(v)),
+                    "nhqg_stream_copy")
+        return v.value
 
     def device_bytes(self):
         v = C.c_uint64()

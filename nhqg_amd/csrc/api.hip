@@ -590,7 +590,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(field(c, &c->spw, Ng));
   RUN(field(c, &c->spth, Ng));
   RUN(dalloc(c, &c->partial, (size_t)Nq * c->Pred + 64));
-  RUN(dalloc(c, &c->sbp, (size_t)Nq * 16 + 2 * (size_t)Nq * Nq));
+  RUN(dalloc(c, &c->sbp, (size_t)Nq * 16 + (size_t)Nq * Nq));
   RUN(dalloc(c, &c->g, (size_t)Nq));
   c->fstride = (size_t)Nq * c->Nx * c->nky;
   RUN(dalloc(c, &c->X5, c->fstride * 8));
@@ -957,6 +957,38 @@ int nhqg_profile_step(nhqg_ctx* c, double* out, int32_t n_out) {
 
 const char* nhqg_timing_name(int32_t i) {
   return (i >= 0 && i < CL_COUNT) ? kClassNames[i] : "";
+}
+
+int nhqg_stream_copy(nhqg_ctx* c, uint64_t bytes, int32_t reps, double* gbps) {
+  if (!c || !gbps || reps < 1 || bytes < 16) return NHQG_ERR_ARG;
+  const size_t n16 = (size_t)(bytes / 16);
+  void *a = nullptr, *b = nullptr;
+  CK(hipMalloc(&a, n16 * 16));
+  hipError_t e = hipMalloc(&b, n16 * 16);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return NHQG_ERR_HIP;
+  }
+  int r = NHQG_OK;
+  float ms = 0.0f;
+  e = hipMemsetAsync(a, 0, n16 * 16, c->st);
+  if (e == hipSuccess) e = launch_stream_copy(n16, (const double2*)a, (double2*)b, c->st);
+  if (e == hipSuccess) e = hipEventRecord(c->ev0, c->st);
+  for (int i = 0; e == hipSuccess && i < reps; ++i)
+    e = launch_stream_copy(n16, (const double2*)a, (double2*)b, c->st);
+  if (e == hipSuccess) e = hipEventRecord(c->ev1, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  if (e != hipSuccess) {
+    c->err = std::string("stream copy: ") + hipGetErrorString(e);
+    r = NHQG_ERR_HIP;
+  } else {
+    *gbps = 2.0 * n16 * 16.0 * reps / (ms * 1e-3) / 1e9;
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return r;
 }
 
 int nhqg_device_bytes(nhqg_ctx* c, uint64_t* bytes) {

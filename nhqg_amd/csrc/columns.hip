@@ -130,11 +130,30 @@ hipError_t launch_parseval(int nl, int nm, int ld, int P, const double* a, const
 // recurrences in registers).  Each substep is then three small parallel
 // matrix-vector products.  d1b/lb hold the tridiagonal SBP D1 and
 // Laplacian as [i*3 + (j-i+1)].
-// sbp buffer (doubles, n = Nz+1): [0,5n) B band, [5n,10n) LU band,
-// [10n,11n) <w^2>, [11n,12n) forward-substitution scratch (n*n follows in
-// ainv), then ainv: n*n row-major A^{-1}, then ys: n*n scratch.
 // ---------------------------------------------------------------------------
 #define SBP_NMAX 272
+
+// y[i] = sum_j M[i][j] x[j] for a small dense row-major M, one wavefront per
+// row: lanes stride the row (coalesced) and a fixed xor-shuffle tree sums
+// the lane partials, so the result does not depend on scheduling.
+NHQG_DEV double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+NHQG_DEV void block_matvec(int n_out, int n_in, const double* __restrict__ M, const double* x,
+                           double* y) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int i = wave; i < n_out; i += nw) {
+    const double* r = M + (size_t)i * n_in;
+    double t = 0.0;
+    for (int j = lane; j < n_in; j += 64) t += r[j] * x[j];
+    t = wave_sum(t);
+    if (lane == 0) y[i] = t;
+  }
+}
+
 
 NHQG_DEV double band3(const double* b3, int i, int j, int n) {
   const int o = j - i;
@@ -145,7 +164,7 @@ NHQG_DEV double band3(const double* b3, int i, int j, int n) {
 __global__ void __launch_bounds__(256)
 sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
                    const double* __restrict__ lb, double c1, double c2, double* __restrict__ Bg,
-                   double* __restrict__ ainv, double* __restrict__ ys) {
+                   double* __restrict__ LUg) {
   __shared__ double w2[SBP_NMAX];
   __shared__ double LU[SBP_NMAX * 5];
   const int tid = threadIdx.x;
@@ -184,54 +203,60 @@ sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const doubl
     }
   }
   __syncthreads();
-  // column c of A^{-1}: L y = e_c, U x = y; ys/ainv rows indexed by c
-  for (int c = tid; c < n; c += 256) {
-    double* y = ys + (size_t)c * n;
-    double y1 = 0.0, y2 = 0.0;  // y[i-1], y[i-2]
-    for (int i = 0; i < n; ++i) {
-      double v = (i == c) ? 1.0 : 0.0;
-      if (i >= 1) v -= LU[i * 5 + 1] * y1;
-      if (i >= 2) v -= LU[i * 5 + 0] * y2;
-      y[i] = v;
-      y2 = y1;
-      y1 = v;
-    }
-    double x1 = 0.0, x2 = 0.0;  // x[i+1], x[i+2]
-    for (int i = n - 1; i >= 0; --i) {
-      double v = y[i];
-      if (i + 1 < n) v -= LU[i * 5 + 3] * x1;
-      if (i + 2 < n) v -= LU[i * 5 + 4] * x2;
-      v /= LU[i * 5 + 2];
-      ainv[(size_t)i * n + c] = v;
-      x2 = x1;
-      x1 = v;
-    }
+  for (int i = tid; i < 5 * n; i += 256) LUg[i] = LU[i];
+}
+
+// Columns of A^{-1} from the band LU: thread c solves L y = e_c, U x = y
+// with y kept in LDS; writes column c of the row-major A^{-1}.
+#define SBP_COLS 16
+__global__ void __launch_bounds__(SBP_COLS)
+sbp_inverse_kernel(int n, const double* __restrict__ LUg, double* __restrict__ ainv) {
+  __shared__ double LU[SBP_NMAX * 5];
+  __shared__ double y[SBP_COLS][SBP_NMAX];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 5 * n; i += SBP_COLS) LU[i] = LUg[i];
+  __syncthreads();
+  const int c = blockIdx.x * SBP_COLS + tid;
+  if (c >= n) return;
+  double y1 = 0.0, y2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double v = (i == c) ? 1.0 : 0.0;
+    if (i >= 1) v -= LU[i * 5 + 1] * y1;
+    if (i >= 2) v -= LU[i * 5 + 0] * y2;
+    y[tid][i] = v;
+    y2 = y1;
+    y1 = v;
+  }
+  double x1 = 0.0, x2 = 0.0;
+  for (int i = n - 1; i >= 0; --i) {
+    double v = y[tid][i];
+    if (i + 1 < n) v -= LU[i * 5 + 3] * x1;
+    if (i + 2 < n) v -= LU[i * 5 + 4] * x2;
+    v /= LU[i * 5 + 2];
+    ainv[(size_t)i * n + c] = v;
+    x2 = x1;
+    x1 = v;
   }
 }
 
 // One substep's 1-D solve: flux -> rhs -> thb_sbp_new = A^{-1} rhs -> g, thb.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
                    const double* __restrict__ thb2s, const double* __restrict__ s2thb,
                    double mu_sdt, const double* __restrict__ Bg, const double* __restrict__ ainv,
                    double* __restrict__ thb_io, double* __restrict__ g_out) {
   __shared__ double fl[SBP_NMAX], tb[SBP_NMAX], rh[SBP_NMAX], nw[SBP_NMAX], th0[SBP_NMAX];
   const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += 256) th0[i] = thb_io[i];
-  for (int i = tid; i < n; i += 256) {
+  for (int i = tid; i < n; i += blockDim.x) th0[i] = thb_io[i];
+  for (int i = tid; i < n; i += blockDim.x) {
     double s = 0.0;
     for (int p = 0; p < P; ++p) s += partial[(size_t)i * P + p];
     fl[i] = s;
   }
   __syncthreads();
-  for (int i = tid; i < n; i += 256) {
-    double t = 0.0;
-    const double* r = thb2s + (size_t)i * n;
-    for (int j = 0; j < n; ++j) t += r[j] * th0[j];
-    tb[i] = t;
-  }
+  block_matvec(n, n, thb2s, th0, tb);
   __syncthreads();
-  for (int i = tid; i < n; i += 256) {
+  for (int i = tid; i < n; i += blockDim.x) {
     double r = 0.0;
     for (int o = -2; o <= 2; ++o) {
       const int j = i + o;
@@ -246,14 +271,9 @@ sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const doubl
     rh[i] = (i == 0 || i == n - 1) ? 0.0 : r;
   }
   __syncthreads();
-  for (int i = tid; i < n; i += 256) {
-    double t = 0.0;
-    const double* r = ainv + (size_t)i * n;
-    for (int j = 0; j < n; ++j) t += r[j] * rh[j];
-    nw[i] = t;
-  }
+  block_matvec(n, n, ainv, rh, nw);
   __syncthreads();
-  for (int i = tid; i < n; i += 256) {
+  for (int i = tid; i < n; i += blockDim.x) {
     double gs = 0.0;
     for (int o = -1; o <= 1; ++o) {
       const int j = i + o;
@@ -267,23 +287,22 @@ sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const doubl
     nw[n - 1] = 0.0;
   }
   __syncthreads();
-  for (int i = tid; i < n; i += 256) {
-    double t = 0.0;
-    const double* r = s2thb + (size_t)i * n;
-    for (int j = 0; j < n; ++j) t += r[j] * nw[j];
-    thb_io[i] = t;
-  }
+  block_matvec(n, n, s2thb, nw, thb_io);
 }
 
+// sbp buffer (doubles, n = Nz+1): [0,5n) B band, [5n,10n) LU band,
+// [16n, 16n + n*n) A^{-1} row-major.
 hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double* d1b,
                               const double* lb, double c1, double c2, double* sbp,
                               hipStream_t s) {
   if (n > SBP_NMAX) return hipErrorInvalidValue;
   double* Bg = sbp;
+  double* LUg = sbp + 5 * (size_t)n;
   double* ainv = sbp + 16 * (size_t)n;
-  double* ys = ainv + (size_t)n * n;
   hipLaunchKernelGGL(sbp_prepare_kernel, dim3(1), dim3(256), 0, s, n, P, partial, d1b, lb, c1, c2,
-                     Bg, ainv, ys);
+                     Bg, LUg);
+  hipLaunchKernelGGL(sbp_inverse_kernel, dim3((n + SBP_COLS - 1) / SBP_COLS), dim3(SBP_COLS), 0, s,
+                     n, LUg, ainv);
   return hipGetLastError();
 }
 
@@ -293,7 +312,7 @@ hipError_t launch_sbp_substep(int n, int P, const double* partial, const double*
   if (n > SBP_NMAX) return hipErrorInvalidValue;
   const double* Bg = sbp;
   const double* ainv = sbp + 16 * (size_t)n;
-  hipLaunchKernelGGL(sbp_substep_kernel, dim3(1), dim3(256), 0, s, n, P, partial, d1b, thb2s,
+  hipLaunchKernelGGL(sbp_substep_kernel, dim3(1), dim3(1024), 0, s, n, P, partial, d1b, thb2s,
                      s2thb, mu_sdt, Bg, ainv, thb_io, g_out);
   return hipGetLastError();
 }
@@ -317,14 +336,10 @@ hipError_t launch_sbp_update(int n, int ld, double* th, const double* w, const d
 }
 
 // y = M x for small dense M (n_out x n_in); one block.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 small_matvec_kernel(int n_out, int n_in, const double* __restrict__ M, const double* __restrict__ x,
                     double* __restrict__ y) {
-  for (int i = threadIdx.x; i < n_out; i += 256) {
-    double t = 0.0;
-    for (int j = 0; j < n_in; ++j) t += M[(size_t)i * n_in + j] * x[j];
-    y[i] = t;
-  }
+  block_matvec(n_out, n_in, M, x, y);
 }
 
 // y = x0 + c * (x1 - x2) (n entries; one block).
@@ -335,7 +350,7 @@ small_axpy_kernel(int n, const double* x0, const double* x1, const double* x2, d
 
 hipError_t launch_small_matvec(int n_out, int n_in, const double* M, const double* x, double* y,
                                hipStream_t s) {
-  hipLaunchKernelGGL(small_matvec_kernel, dim3(1), dim3(256), 0, s, n_out, n_in, M, x, y);
+  hipLaunchKernelGGL(small_matvec_kernel, dim3(1), dim3(1024), 0, s, n_out, n_in, M, x, y);
   return hipGetLastError();
 }
 
@@ -482,5 +497,18 @@ hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mo
                        const double* inv_denom, const double2* q, double2* eq, hipStream_t s) {
   hipLaunchKernelGGL(beta_kernel, dim3((nm + 255) / 256, nl), dim3(256), 0, s, nm, ldc, beta,
                      kx_mode, inv_denom, q, eq);
+  return hipGetLastError();
+}
+
+// Plain streaming copy (calibration of the HBM traffic counters and a
+// practical bandwidth ceiling): dst[i] = src[i], 16-byte elements.
+__global__ void __launch_bounds__(256)
+stream_copy_kernel(size_t n, const double2* __restrict__ src, double2* __restrict__ dst) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+hipError_t launch_stream_copy(size_t n16, const double2* src, double2* dst, hipStream_t s) {
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, s, n16, src, dst);
   return hipGetLastError();
 }

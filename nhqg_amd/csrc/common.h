@@ -119,6 +119,7 @@ hipError_t launch_gather(int nl, int nm, int nm_pad, int full_n, const int* full
 hipError_t launch_scatter(int nl, int nm, int nm_pad, int full_n, const int* full_idx,
                           const double2* comp, double2* full, hipStream_t s);
 hipError_t launch_max_reduce(size_t n, const double* x, double* out, hipStream_t s);
+hipError_t launch_stream_copy(size_t n16, const double2* src, double2* dst, hipStream_t s);
 hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mode,
                        const double* inv_denom, const double2* q, double2* eq, hipStream_t s);
 
