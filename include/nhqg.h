@@ -39,7 +39,7 @@ typedef struct nhqg_ctx nhqg_ctx;
  * derivations, nhqg/grid.py:669-687). */
 typedef struct {
   int32_t nx;          /* horizontal grid points, power of two 16..1024 */
-  int32_t nz;          /* Chebyshev intervals; Nz+1 CGL levels, Nz <= 271 */
+  int32_t nz;          /* Chebyshev intervals; Nz+1 CGL levels, Nz <= 256 */
   int32_t nkx, nky;    /* kept kx rows / ky columns of the mode layout */
   int32_t n_shells;    /* number of |k|^2 shells in shell_inv */
   int32_t n_pairs;     /* entries of herm_pairs */

@@ -16,6 +16,9 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep packed-FP32 instructions out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: without it the contraction's
+    # accumulators are copied between VGPRs and AGPRs around loop edges
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

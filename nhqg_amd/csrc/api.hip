@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cmath>
 #include <algorithm>
+#include <initializer_list>
 
 #include "common.h"
 #include "../../include/nhqg.h"
@@ -19,8 +20,11 @@ namespace {
 std::string g_create_error;
 
 struct Ops {
-  double *V, *VSw, *VSd, *Vinv, *Pw, *Pd, *Qw, *GSw, *W2S, *T2S, *S2T, *thb2s, *s2thb, *proj_d;
-  double *d1b, *lb, *Mtw, *Mwt, *Mth2w, *cc, *row_cc, *cc_sbp, *Sw, *Sd;
+  // contraction operators, packed k-major (GemmOp); Psbp = T2S * S2T is one
+  // full SBP substep cycle (SBP grid -> Galerkin theta -> SBP grid)
+  GemmOp V, VSw, VSd, Vinv, Pw, Pd, Qw, GSw, W2S, T2S, S2T, Psbp, Mtw, Mwt, Mth2w, Sw, Sd, row_cc;
+  double *thb2s, *s2thb, *proj_d;
+  double *d1b, *lb, *cc, *cc_sbp;
   int *rows, *kidx_of_x, *full_idx, *partner, *shell_ptr, *shell_modes;
   int2* pairs;
   double *kx_row, *ky_col, *kx_mode, *ksq, *inv_denom, *iaq, *iath, *idq, *diss_q, *diss_w,
@@ -53,7 +57,7 @@ struct nhqg_ctx {
   double *Rq, *Rw, *Rth, *Rthb;
   double *s1q, *s1w, *s1th, *s1thb, *c1th, *c1thb;
   double *rhs_w, *tmpg;
-  double *wsbp, *thsbp;
+  double *wsbp, *thsbp, *thsbp2;
   double *spq, *spw, *spth;
   double *partial, *sbp, *g;
   double2 *X5, *Y6;
@@ -109,6 +113,19 @@ int upload_modes(nhqg_ctx* c, double** p, const double* host) {
   std::vector<double> v((size_t)c->nm_pad, 0.0);
   if (host) std::memcpy(v.data(), host, sizeof(double) * c->nm);
   return upload(c, p, v.data(), v.size());
+}
+
+// contraction operator (M x K row-major on the host) packed for
+// launch_gemm_batch with mt row tiles
+int upload_op(nhqg_ctx* c, GemmOp* op, const double* host, int M, int K, int mt) {
+  *op = GemmOp{nullptr, M, K, mt, gemm_op_lda(mt), gemm_op_kp(K)};
+  if (!host) return NHQG_OK;
+  std::vector<double> v((size_t)op->lda * op->kp);
+  gemm_pack_op(host, M, K, mt, v.data());
+  double* d = nullptr;
+  int r = upload(c, &d, v.data(), v.size());
+  op->At = d;
+  return r;
 }
 
 int field(nhqg_ctx* c, double** p, int nl) {
@@ -168,10 +185,45 @@ void prof_end(nhqg_ctx* c, int cls, hipEvent_t a, double work) {
     if (c->prof) prof_end(c, (cls), pa_, (work));   \
   } while (0)
 
-int gemm(nhqg_ctx* c, int M, int K, const double* A, const double* B, double* C, const Epi& e) {
-  PL(CL_GEMM, 4.0 * M * K * (double)c->nm,
-     launch_gemm_f64(M, c->ld, K, A, K, B, C, c->ld, e, c->st));
+// One contraction of a batch (launch_gemm_batch); W/g/bc select the fused
+// SBP substep update of the input.
+struct Job {
+  const GemmOp* op;
+  const double* B;
+  double* C;
+  Epi e;
+  const double* W = nullptr;
+  const double* g = nullptr;
+  double bc = 0.0;
+};
+
+int gemm_jobs(nhqg_ctx* c, std::initializer_list<Job> jobs) {
+  GemmBatch b;
+  std::memset(&b, 0, sizeof(b));
+  b.ld = c->ld;
+  int n = 0;
+  double work = 0.0;
+  for (const Job& j : jobs) {
+    if (n == GEMM_MAX_JOBS) {
+      c->err = "too many contractions in one batch";
+      return NHQG_ERR_ARG;
+    }
+    GemmJob& g = b.job[n++];
+    g.op = *j.op;
+    g.B = j.B;
+    g.C = j.C;
+    g.W = j.W;
+    g.g = j.g;
+    g.bc = j.bc;
+    g.e = j.e;
+    work += 4.0 * j.op->M * j.op->K * (double)c->nm;
+  }
+  PL(CL_GEMM, work, launch_gemm_batch(b, n, c->st));
   return NHQG_OK;
+}
+
+int gemm(nhqg_ctx* c, const GemmOp& op, const double* B, double* C, const Epi& e) {
+  return gemm_jobs(c, {Job{&op, B, C, e}});
 }
 
 int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
@@ -190,9 +242,8 @@ int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
 int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, double* Eq,
                  double* Ew, double* Eth) {
   const Ops& o = c->o;
-  RUN(gemm(c, c->Nq, c->Nq, o.V, q, c->qn, epi_make(1.0)));
-  RUN(gemm(c, c->Nq, c->Ng, o.VSw, w, c->wn, epi_make(1.0)));
-  RUN(gemm(c, c->Nq, c->Ng, o.VSd, th, c->thn, epi_make(1.0)));
+  RUN(gemm_jobs(c, {Job{&o.V, q, c->qn, epi_make(1.0)}, Job{&o.VSw, w, c->wn, epi_make(1.0)},
+                    Job{&o.VSd, th, c->thn, epi_make(1.0)}}));
   FluxArgs a;
   a.N = c->Nx;
   a.nq = c->Nq;
@@ -237,12 +288,11 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
     epi_add(ew, -1.0, c->spw);
     epi_add(et, -1.0, c->spth);
   }
-  RUN(gemm(c, c->Nq, c->Nq, o.Vinv, c->qn, Eq, eq));
+  RUN(gemm_jobs(c, {Job{&o.Vinv, c->qn, Eq, eq}, Job{&o.Pw, c->wn, Ew, ew},
+                    Job{&o.Pd, c->thn, Eth, et}}));
   if (c->P.beta != 0.0)
     LK(launch_beta(c->Nq, c->ldc, c->nm, c->P.beta, o.kx_mode, o.inv_denom, (const double2*)q,
                    (double2*)Eq, c->st));
-  RUN(gemm(c, c->Ng, c->Nq, o.Pw, c->wn, Ew, ew));
-  RUN(gemm(c, c->Ng, c->Nq, o.Pd, c->thn, Eth, et));
   return NHQG_OK;
 }
 
@@ -262,17 +312,17 @@ int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double
     e.s_acc = o.idq;
     epi_add(e, 1.0, Rw);
     epi_add(e, ars, Rth, o.iath);
-    RUN(gemm(c, c->Ng, c->Nq, o.Qw, Rq, c->rhs_w, e));
+    RUN(gemm(c, o.Qw, Rq, c->rhs_w, e));
     PL(CL_SHELL, shell_bytes(c),
        launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
                           c->rhs_w, ow, Rth, oth, o.iath, al, c->st));
   } else {
-    RUN(gemm(c, c->Ng, c->Ng, o.Mth2w, Rth, c->tmpg, epi_make(1.0)));
+    RUN(gemm(c, o.Mth2w, Rth, c->tmpg, epi_make(1.0)));
     Epi e = epi_make(al);
     e.s_acc = o.idq;
     epi_add(e, 1.0, Rw);
     epi_add(e, ars, c->tmpg, o.iath);
-    RUN(gemm(c, c->Ng, c->Nq, o.Qw, Rq, c->rhs_w, e));
+    RUN(gemm(c, o.Qw, Rq, c->rhs_w, e));
     // the fused theta output is overwritten below with the mapped form
     PL(CL_SHELL, shell_bytes(c),
        launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
@@ -280,17 +330,22 @@ int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double
     Epi et = epi_make(al);
     epi_add(et, 1.0, Rth);
     et.s_out = o.iath;
-    RUN(gemm(c, c->Ng, c->Ng, o.Mwt, ow, oth, et));
+    RUN(gemm(c, o.Mwt, ow, oth, et));
   }
   Epi eq = epi_make(al);
   epi_add(eq, 1.0, Rq);
   eq.s_out = o.iaq;
-  RUN(gemm(c, c->Nq, c->Ng, o.GSw, ow, oq, eq));
+  RUN(gemm(c, o.GSw, ow, oq, eq));
   return NHQG_OK;
 }
 
 // SBP2 thermal corrector with hoisted invariants (nhqg/solver.py:909-950).
 // th_out may equal th_in; thb_out is written (thb_in copied first).
+// The substeps stay on the SBP grid: each one maps its updated field back
+// and forth with the single operator Psbp = T2S * S2T (the reference applies
+// S2T then T2S), and the update th - sdt * w_sbp * g with zeroed boundary
+// rows is applied as the contraction reads its input; only the last substep
+// returns to Galerkin coefficients (S2T).
 int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double* thb_in,
               double* th_out, double* thb_out, double total_dt) {
   const Ops& o = c->o;
@@ -298,7 +353,10 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
   const int ns = c->P.substeps > 0 ? c->P.substeps : 1;
   const double sdt = total_dt / ns;
   const double mu = c->P.eps_sq, kappa = 1.0 / c->P.sigma;
-  RUN(gemm(c, n, c->Ng, o.W2S, w_in, c->wsbp, epi_make(1.0)));
+  double* cur = c->thsbp;
+  double* nxt = c->thsbp2;
+  RUN(gemm_jobs(c, {Job{&o.W2S, w_in, c->wsbp, epi_make(1.0)},
+                    Job{&o.T2S, th_in, cur, epi_make(1.0)}}));
   PL(CL_PARSEVAL, 16.0 * n * (double)c->nm,
      launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->wsbp, o.pw, o.partner, c->partial,
                      c->st));
@@ -306,19 +364,19 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
                         0.25 * mu * sdt * sdt, c->sbp, c->st));
   if (thb_out != thb_in)
     PL(CL_SMALL, 0.0, launch_small_axpy(n, thb_in, thb_in, thb_in, 0.0, thb_out, c->st));
-  const double* cur = th_in;
   for (int s = 0; s < ns; ++s) {
-    RUN(gemm(c, n, c->Ng, o.T2S, cur, c->thsbp, epi_make(1.0)));
     PL(CL_PARSEVAL, 32.0 * n * (double)c->nm,
-       launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->thsbp, o.pw, o.partner, c->partial,
+       launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, cur, o.pw, o.partner, c->partial,
                        c->st));
     PL(CL_SBP, 0.0,
        launch_sbp_substep(n, c->Pred, c->partial, o.d1b, o.thb2s, o.s2thb, mu * sdt, c->sbp,
                           thb_out, c->g, c->st));
-    PL(CL_ELEMENTWISE, 48.0 * n * (double)c->nm,
-       launch_sbp_update(n, c->ld, c->thsbp, c->wsbp, c->g, sdt, c->st));
-    RUN(gemm(c, c->Ng, n, o.S2T, c->thsbp, th_out, epi_make(1.0)));
-    cur = th_out;
+    Job j{s + 1 < ns ? &o.Psbp : &o.S2T, cur, s + 1 < ns ? nxt : th_out, epi_make(1.0)};
+    j.W = c->wsbp;
+    j.g = c->g;
+    j.bc = sdt;
+    RUN(gemm_jobs(c, {j}));
+    std::swap(cur, nxt);
   }
   return NHQG_OK;
 }
@@ -382,11 +440,10 @@ int enqueue_step(nhqg_ctx* c) {
     epi_add(e, dt * delta, c->E1q);
     epi_add(e, dt * (1.0 - delta), c->E2q);
     epi_add(e, -om, c->s1q, o.diss_q);
-    RUN(gemm(c, Nq, Ng, o.GSw, c->s1w, c->Rq, e));
     // R_w2 = ... + om*(inv_denom Qw q1 + Ra_s Mtw th1) - om diss_w w1
     const double* thw = th1;
     if (c->P.w_neumann) {
-      RUN(gemm(c, Ng, Ng, o.Mtw, th1, c->tmpg, epi_make(1.0)));
+      RUN(gemm(c, o.Mtw, th1, c->tmpg, epi_make(1.0)));
       thw = c->tmpg;
     }
     Epi f = epi_make(om);
@@ -396,11 +453,11 @@ int enqueue_step(nhqg_ctx* c) {
     epi_add(f, dt * (1.0 - delta), c->E2w);
     epi_add(f, om * c->P.ra_sigma, thw);
     epi_add(f, -om, c->s1w, o.diss_w);
-    RUN(gemm(c, Ng, Nq, o.Qw, c->s1q, c->Rw, f));
+    RUN(gemm_jobs(c, {Job{&o.GSw, c->s1w, c->Rq, e}, Job{&o.Qw, c->s1q, c->Rw, f}}));
     // R_th2 = ... + om*I_th - om diss_th th1 + om*(th1 - th1p)/alpha
     const double* ith = c->s1w;
     if (c->P.w_neumann) {
-      RUN(gemm(c, Ng, Ng, o.Mwt, c->s1w, c->tmpg, epi_make(1.0)));
+      RUN(gemm(c, o.Mwt, c->s1w, c->tmpg, epi_make(1.0)));
       ith = c->tmpg;
     }
     Epi h = epi_make(0.0);
@@ -451,8 +508,8 @@ int check_params(const nhqg_params* p, std::string& err) {
     err = "nx must be a power of two in [16, 1024]";
     return NHQG_ERR_ARG;
   }
-  if (p->nz < 3 || p->nz + 1 > 272) {
-    err = "nz must be in [3, 271]";
+  if (p->nz < 3 || p->nz > 256) {
+    err = "nz must be in [3, 256]";
     return NHQG_ERR_ARG;
   }
   if (p->nkx < 1 || p->nkx > nx || p->nky < 1 || p->nky > nx / 2 + 1) {
@@ -479,7 +536,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   c->nkx = p->nkx;
   c->nky = p->nky;
   c->nm = p->nkx * p->nky;
-  c->nm_pad = (c->nm + 31) / 32 * 32;
+  c->nm_pad = (c->nm + 63) / 64 * 64;
   c->ld = 2 * c->nm_pad;
   c->ldc = c->nm_pad;
   c->Pred = 16;
@@ -488,37 +545,46 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   CK(hipEventCreate(&c->ev1));
   const int Nq = c->Nq, Ng = c->Ng, nm = c->nm;
   Ops& o = c->o;
-  RUN(upload(c, &o.V, h->V, (size_t)Nq * Nq));
-  RUN(upload(c, &o.VSw, h->VSw, (size_t)Nq * Ng));
-  RUN(upload(c, &o.VSd, h->VSd, (size_t)Nq * Ng));
-  RUN(upload(c, &o.Vinv, h->Vinv, (size_t)Nq * Nq));
-  RUN(upload(c, &o.Pw, h->Pw, (size_t)Ng * Nq));
-  RUN(upload(c, &o.Pd, h->Pd, (size_t)Ng * Nq));
-  RUN(upload(c, &o.Qw, h->Qw, (size_t)Ng * Nq));
-  RUN(upload(c, &o.GSw, h->GSw, (size_t)Nq * Ng));
-  RUN(upload(c, &o.W2S, h->W2S, (size_t)Nq * Ng));
-  RUN(upload(c, &o.T2S, h->T2S, (size_t)Nq * Ng));
-  RUN(upload(c, &o.S2T, h->S2T, (size_t)Ng * Nq));
+  const int mt = gemm_row_tiles(Nq);
+  RUN(upload_op(c, &o.V, h->V, Nq, Nq, mt));
+  RUN(upload_op(c, &o.VSw, h->VSw, Nq, Ng, mt));
+  RUN(upload_op(c, &o.VSd, h->VSd, Nq, Ng, mt));
+  RUN(upload_op(c, &o.Vinv, h->Vinv, Nq, Nq, mt));
+  RUN(upload_op(c, &o.Pw, h->Pw, Ng, Nq, mt));
+  RUN(upload_op(c, &o.Pd, h->Pd, Ng, Nq, mt));
+  RUN(upload_op(c, &o.Qw, h->Qw, Ng, Nq, mt));
+  RUN(upload_op(c, &o.GSw, h->GSw, Nq, Ng, mt));
+  RUN(upload_op(c, &o.W2S, h->W2S, Nq, Ng, mt));
+  RUN(upload_op(c, &o.T2S, h->T2S, Nq, Ng, mt));
+  RUN(upload_op(c, &o.S2T, h->S2T, Ng, Nq, mt));
+  {
+    // one SBP substep cycle: Psbp = T2S * S2T (Nq x Nq)
+    std::vector<double> P((size_t)Nq * Nq, 0.0);
+    for (int i = 0; i < Nq; ++i)
+      for (int k = 0; k < Ng; ++k) {
+        const double a = h->T2S[(size_t)i * Ng + k];
+        for (int j = 0; j < Nq; ++j) P[(size_t)i * Nq + j] += a * h->S2T[(size_t)k * Nq + j];
+      }
+    RUN(upload_op(c, &o.Psbp, P.data(), Nq, Nq, mt));
+  }
   RUN(upload(c, &o.thb2s, h->thb2s, (size_t)Nq * Nq));
   RUN(upload(c, &o.s2thb, h->s2thb, (size_t)Nq * Nq));
   RUN(upload(c, &o.proj_d, h->proj_d, (size_t)Nq * Nq));
   RUN(upload(c, &o.d1b, h->d1_band, (size_t)Nq * 3));
   RUN(upload(c, &o.lb, h->lap_band, (size_t)Nq * 3));
   RUN(upload(c, &o.cc, h->cc, (size_t)Nq));
-  RUN(upload(c, &o.row_cc, h->row_cc, (size_t)Nq));
+  RUN(upload_op(c, &o.row_cc, h->row_cc, 1, Nq, 1));
   RUN(upload(c, &o.cc_sbp, h->cc_sbp, (size_t)Nq));
-  RUN(upload(c, &o.Sw, h->Sw, (size_t)Nq * Ng));
-  RUN(upload(c, &o.Sd, h->Sd, (size_t)Nq * Ng));
+  RUN(upload_op(c, &o.Sw, h->Sw, Nq, Ng, mt));
+  RUN(upload_op(c, &o.Sd, h->Sd, Nq, Ng, mt));
   if (p->w_neumann) {
     if (!h->Mtw || !h->Mwt || !h->Mth2w) {
       c->err = "w_neumann requires Mtw, Mwt and Mth2w";
       return NHQG_ERR_ARG;
     }
-    RUN(upload(c, &o.Mtw, h->Mtw, (size_t)Ng * Ng));
-    RUN(upload(c, &o.Mwt, h->Mwt, (size_t)Ng * Ng));
-    RUN(upload(c, &o.Mth2w, h->Mth2w, (size_t)Ng * Ng));
-  } else {
-    o.Mtw = o.Mwt = o.Mth2w = nullptr;
+    RUN(upload_op(c, &o.Mtw, h->Mtw, Ng, Ng, mt));
+    RUN(upload_op(c, &o.Mwt, h->Mwt, Ng, Ng, mt));
+    RUN(upload_op(c, &o.Mth2w, h->Mth2w, Ng, Ng, mt));
   }
   RUN(upload(c, &o.rows, h->rows, (size_t)c->nkx));
   RUN(upload(c, &o.kidx_of_x, h->kidx_of_x, (size_t)c->Nx));
@@ -586,6 +652,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(field(c, &c->tmpg, Ng));
   RUN(field(c, &c->wsbp, Nq));
   RUN(field(c, &c->thsbp, Nq));
+  RUN(field(c, &c->thsbp2, Nq));
   RUN(field(c, &c->spq, Nq));
   RUN(field(c, &c->spw, Ng));
   RUN(field(c, &c->spth, Ng));
@@ -649,7 +716,7 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
     return NHQG_OK;
   };
   // nodal q (psi = -q inv_denom, applied through the per-mode weights)
-  RUN(gemm(c, Nq, Nq, o.V, c->yq, c->qn, epi_make(1.0)));
+  RUN(gemm(c, o.V, c->yq, c->qn, epi_make(1.0)));
   LK(launch_parseval(Nq, c->nm, c->ld, P, c->qn, c->qn, o.pw_ke, nullptr, c->partial, c->st));
   RUN(fetch(Nq));
   const double ke_tot = 0.5 * reduce_partials(part, Nq, P, cc.data());
@@ -657,7 +724,7 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
   RUN(fetch(Nq));
   const double q2 = reduce_partials(part, Nq, P, cc.data());
   // barotropic: depth average of nodal q, then the same weights
-  RUN(gemm(c, 1, Nq, o.row_cc, c->qn, c->E2q, epi_make(1.0)));
+  RUN(gemm(c, o.row_cc, c->qn, c->E2q, epi_make(1.0)));
   LK(launch_parseval(1, c->nm, c->ld, P, c->E2q, c->E2q, o.pw_ke, nullptr, c->partial, c->st));
   RUN(fetch(1));
   const double ke_bt = 0.5 * reduce_partials(part, 1, P, nullptr);
@@ -687,8 +754,8 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
   CK(hipStreamSynchronize(c->st));
   CK(hipMemcpy(&max_speed, c->diag, sizeof(double), hipMemcpyDeviceToHost));
   // w, theta nodal
-  RUN(gemm(c, Nq, Ng, o.VSw, c->yw, c->wn, epi_make(1.0)));
-  RUN(gemm(c, Nq, Ng, o.VSd, c->yth, c->thn, epi_make(1.0)));
+  RUN(gemm_jobs(c, {Job{&o.VSw, c->yw, c->wn, epi_make(1.0)},
+                    Job{&o.VSd, c->yth, c->thn, epi_make(1.0)}}));
   LK(launch_parseval(Nq, c->nm, c->ld, P, c->wn, c->thn, o.pw, nullptr, c->partial, c->st));
   RUN(fetch(Nq));
   const double vol_tw = reduce_partials(part, Nq, P, cc.data());
@@ -706,8 +773,8 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
     const double* wl = cc.data();
     std::vector<double> ccs(Nq);
     if (c->P.evolve_mean) {
-      RUN(gemm(c, Nq, Ng, o.W2S, c->yw, c->wsbp, epi_make(1.0)));
-      RUN(gemm(c, Nq, Ng, o.T2S, c->yth, c->thsbp, epi_make(1.0)));
+      RUN(gemm_jobs(c, {Job{&o.W2S, c->yw, c->wsbp, epi_make(1.0)},
+                        Job{&o.T2S, c->yth, c->thsbp, epi_make(1.0)}}));
       LK(launch_parseval(Nq, c->nm, c->ld, P, c->wsbp, c->thsbp, o.pw, o.partner, c->partial,
                          c->st));
       CK(hipMemcpy(ccs.data(), o.cc_sbp, sizeof(double) * Nq, hipMemcpyDeviceToHost));
@@ -722,8 +789,8 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
   // vertical spectra: raw sums of |f_n|^2 of q, w_cheb, th_cheb
   // (diagnostics.py:583-585)
   std::vector<double> spec(3 * (size_t)Nq, 0.0);
-  RUN(gemm(c, Nq, Ng, o.Sw, c->yw, c->wn, epi_make(1.0)));
-  RUN(gemm(c, Nq, Ng, o.Sd, c->yth, c->thn, epi_make(1.0)));
+  RUN(gemm_jobs(c, {Job{&o.Sw, c->yw, c->wn, epi_make(1.0)},
+                    Job{&o.Sd, c->yth, c->thn, epi_make(1.0)}}));
   const double* srcs[3] = {c->yq, c->wn, c->thn};
   for (int f = 0; f < 3; ++f) {
     LK(launch_parseval(Nq, c->nm, c->ld, P, srcs[f], srcs[f], o.ones, nullptr, c->partial, c->st));
@@ -870,18 +937,18 @@ int nhqg_apply(nhqg_ctx* c, int32_t op, double arg) {
     case NHQG_OP_IMPLICIT_TEND: {
       // I_q = G_Z S_w w; I_w = inv_denom Qw q + Ra_s (Mtw) th; I_th = (Mwt) w
       const Ops& o = c->o;
-      RUN(gemm(c, c->Nq, c->Ng, o.GSw, c->yw, c->E1q, epi_make(1.0)));
+      RUN(gemm(c, o.GSw, c->yw, c->E1q, epi_make(1.0)));
       const double* thw = c->yth;
       if (c->P.w_neumann) {
-        RUN(gemm(c, c->Ng, c->Ng, o.Mtw, c->yth, c->tmpg, epi_make(1.0)));
+        RUN(gemm(c, o.Mtw, c->yth, c->tmpg, epi_make(1.0)));
         thw = c->tmpg;
       }
       Epi f = epi_make(1.0);
       f.s_acc = o.inv_denom;
       epi_add(f, c->P.ra_sigma, thw);
-      RUN(gemm(c, c->Ng, c->Nq, o.Qw, c->yq, c->E1w, f));
+      RUN(gemm(c, o.Qw, c->yq, c->E1w, f));
       if (c->P.w_neumann) {
-        RUN(gemm(c, c->Ng, c->Ng, o.Mwt, c->yw, c->E1th, epi_make(1.0)));
+        RUN(gemm(c, o.Mwt, c->yw, c->E1th, epi_make(1.0)));
       } else {
         Epi h = epi_make(0.0);
         epi_add(h, 1.0, c->yw);

@@ -318,23 +318,6 @@ hipError_t launch_sbp_substep(int n, int P, const double* partial, const double*
 }
 
 // theta_sbp <- Z0(theta_sbp - sdt * w_sbp * g[l]); rows 0 and n-1 zeroed.
-__global__ void __launch_bounds__(256)
-sbp_update_kernel(size_t total, int ld, int n, double* __restrict__ th, const double* __restrict__ w,
-                  const double* __restrict__ g, double sdt) {
-  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= total) return;
-  const int l = (int)(idx / (size_t)ld);
-  th[idx] = (l == 0 || l == n - 1) ? 0.0 : th[idx] - sdt * w[idx] * g[l];
-}
-
-hipError_t launch_sbp_update(int n, int ld, double* th, const double* w, const double* g,
-                             double sdt, hipStream_t s) {
-  const size_t total = (size_t)n * ld;
-  hipLaunchKernelGGL(sbp_update_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                     total, ld, n, th, w, g, sdt);
-  return hipGetLastError();
-}
-
 // y = M x for small dense M (n_out x n_in); one block.
 __global__ void __launch_bounds__(1024)
 small_matvec_kernel(int n_out, int n_in, const double* __restrict__ M, const double* __restrict__ x,

@@ -5,7 +5,7 @@
 // column 2m / 2m+1 hold Re / Im of mode m.  Modes are ordered ky-major,
 // m = ky_i * nkx + kx_i, over the kept rows (|kx| <= Nx/3 for the 2/3 band,
 // or all rows) and kept ky columns; nm_pad rounds the mode count up to a
-// multiple of 32 so every contraction tile is full.  Padding modes carry
+// multiple of 64 so every contraction tile is full.  Padding modes carry
 // zeros and zero per-mode factors.
 #pragma once
 
@@ -105,8 +105,6 @@ hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double*
 hipError_t launch_sbp_substep(int n, int P, const double* partial, const double* d1b,
                               const double* thb2s, const double* s2thb, double mu_sdt,
                               double* sbp, double* thb_io, double* g_out, hipStream_t s);
-hipError_t launch_sbp_update(int n, int ld, double* th, const double* w, const double* g,
-                             double sdt, hipStream_t s);
 hipError_t launch_small_matvec(int n_out, int n_in, const double* M, const double* x, double* y,
                                hipStream_t s);
 hipError_t launch_small_axpy(int n, const double* x0, const double* x1, const double* x2, double c,
@@ -123,9 +121,35 @@ hipError_t launch_stream_copy(size_t n16, const double2* src, double2* dst, hipS
 hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mode,
                        const double* inv_denom, const double2* q, double2* eq, hipStream_t s);
 
-// C[M x N] = epilogue(A[M x K] * B[K x N]); B, C and the epilogue X fields
-// share the row stride ld (doubles); N must be a multiple of 64.
-hipError_t launch_gemm_f64(int M, int N, int K, const double* A, int lda, const double* B,
-                           double* C, int ld, const Epi& e, hipStream_t s);
+// Contraction operator on the device: A (M x K) stored k-major, At[k * lda
+// + r] = A[r][k], zero padded to kp columns (a multiple of the k-chunk) and
+// lda = 16*mt + 8 rows; row 16*mt holds A's 257th row when M = 16*mt + 1.
+struct GemmOp {
+  const double* At;
+  int M, K, mt, lda, kp;
+};
+// One contraction C = epilogue(A * B') of a batch.  B' = B, or with W set
+// the SBP substep update B - bc * W * g[row] with rows 0 and K-1 zeroed.
+struct GemmJob {
+  GemmOp op;
+  const double* B;
+  double* C;
+  const double* W;
+  const double* g;
+  double bc;
+  Epi e;
+};
+#define GEMM_MAX_JOBS 3
+struct GemmBatch {
+  GemmJob job[GEMM_MAX_JOBS];
+  int ld;   // row stride (doubles) of every B, C, W and epilogue field
+};
+int gemm_row_tiles(int M);
+int gemm_op_lda(int mt);
+int gemm_op_kp(int K);
+void gemm_pack_op(const double* A, int M, int K, int mt, double* out);
+// Runs b.job[0..nj) in one launch; every job has the same mt and all or
+// none use the update.  ld must be a multiple of 64.
+hipError_t launch_gemm_batch(const GemmBatch& b, int nj, hipStream_t s);
 // out[l][c] = epilogue(0) over nl rows of ld doubles.
 hipError_t launch_combine(int nl, int ld, double* out, const Epi& e, hipStream_t s);

@@ -3,140 +3,196 @@
 // Every vertical operator of the step (Chebyshev <-> CGL transforms,
 // Galerkin stencils and left inverses, tau projections, d/dZ, SBP transfers)
 // is a small real matrix A (at most 257 x 257) applied to all modes of a
-// level-major field: C = A * B with B a (K x 2*nm_pad) real matrix.  One
-// workgroup owns 64 columns and ALL rows of C, so B is streamed from HBM
-// exactly once; A chunks come from L2.  Each wave computes the full column
-// height (MT tiles of 16 rows) for 16 columns with
-// v_mfma_f64_16x16x4_f64, and the result passes through a linear epilogue
-// that fuses the stage-assembly arithmetic.
+// level-major field: C = A * B with B a (K x 2*nm_pad) real matrix.  A
+// launch runs up to three independent contractions of the same row-tile
+// count (blockIdx.y picks the job), so one launch carries enough work to
+// keep every CU busy while other workgroups are in their prologue or
+// writing results.
+//
+// Per wave: 16 columns of B and the full column height of C -- MT row
+// tiles of 16 on v_mfma_f64_16x16x4_f64 plus, when M = 16*MT + 1, the last
+// row as a VALU dot product -- so each B value is read from HBM once and
+// each A fragment (from LDS) feeds one MFMA.  A is stored k-major and zero
+// padded (GemmOp), so chunk loads are coalesced, the LDS stash is
+// conflict-free and no load needs a bounds branch: B rows beyond K are
+// clamped to row K-1 and meet zero columns of A.  B is prefetched R chunks
+// ahead in a statically indexed register ring, and the A chunk for the next
+// iteration is loaded before the B ring entry, so the in-order vmcnt wait
+// for the A stash leaves the ring in flight.
 #include "common.h"
 
-#define KC 8    // k-chunk staged in LDS per iteration (double buffered)
+#define GK_KC 8   // k-chunk staged in LDS per iteration (double buffered)
+#define GK_NW 4   // waves per workgroup; 16 columns each
+#define GK_R 3    // B ring depth in chunks
 
-// Wave w owns row tiles w, w+4, w+8, ... and all four 16-column tiles of
-// the block, so each A value read from LDS feeds 4 MFMAs and each B value
-// feeds up to MTW.  LDS holds k-major chunks (rows contiguous); the next
-// chunk is fetched into registers while the current one is multiplied.
-template <int MT>
-__global__ void __launch_bounds__(256)
-gemm_f64_kernel(int M, int K, const double* __restrict__ A, int lda,
-                const double* __restrict__ B, double* C, int ld, Epi e) {
+static_assert(GK_KC % 4 == 0, "k-chunk must be a multiple of the MFMA depth");
+
+template <int MT, bool UPD>
+__global__ void __launch_bounds__(GK_NW * 64)
+gemm_f64_kernel(GemmBatch bt) {
+  constexpr int NT = GK_NW * 64;
+  constexpr int KC = GK_KC;
+  constexpr int R = GK_R;
   constexpr int MP = MT * 16;
-  constexpr int SA = MP + 2;          // padded row length of the A chunk
-  constexpr int SB = 64 + 2;
-  constexpr int MTW = (MT + 3) / 4;   // row tiles per wave
-  constexpr int NA = (MP * KC + 255) / 256;
-  constexpr int NB = (KC * 64) / 256;
+  constexpr int SA = MP + ((MT % 2 == 0) ? 16 : 0);   // row pitch: lk rows land on disjoint banks
+  constexpr int NA = (MP * KC + NT - 1) / NT;
+  constexpr bool FULL = (NA * NT == MP * KC);
+  constexpr int KS = KC / 4;
   __shared__ double As[2][KC][SA];
-  __shared__ double Bs[2][KC][SB];
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-  const int lane = tid & 63;
-  const int n0 = blockIdx.x * 64;
-  const int li = lane & 15;
-  const int lk = lane >> 4;
 
-  v4d acc[MTW][4];
-#pragma unroll
-  for (int i = 0; i < MTW; ++i)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[i][c] = v4d{0.0, 0.0, 0.0, 0.0};
+  const GemmJob& J = bt.job[blockIdx.y];
+  const double* __restrict__ At = J.op.At;
+  const int lda = J.op.lda;
+  const int M = J.op.M, K = J.op.K;
+  const int nchunks = J.op.kp / KC;
+  const double* __restrict__ B = J.B;
+  const int ld = bt.ld;
 
-  double ra[NA], rb[NB];
-  auto fetch = [&](int k0) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int col = blockIdx.x * (GK_NW * 16) + wave * 16 + li;
+
+  v4d acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  double last = 0.0;
+  double ra[NA];
+  double bq[R][KS], lq[R][KS];
+  double wq[UPD ? R : 1][KS], gq[UPD ? R : 1][KS];
+
+  auto fetchA = [&](int ch) {
+    const double* src = At + (size_t)ch * KC * lda;
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
-      const int idx = tid + u * 256;        // idx = r * KC + kc (kc fastest)
-      const int r = idx / KC, kc = idx % KC;
-      ra[u] = (idx < MP * KC && r < M && k0 + kc < K) ? A[(size_t)r * lda + k0 + kc] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      const int idx = tid + u * 256;        // idx = kr * 64 + c
-      const int kr = idx >> 6, cc = idx & 63;
-      rb[u] = (k0 + kr < K) ? B[(size_t)(k0 + kr) * ld + n0 + cc] : 0.0;
+      int idx = tid + u * NT;                // idx = kc * MP + r
+      if (!FULL) idx = min(idx, MP * KC - 1);
+      ra[u] = src[(size_t)(idx / MP) * lda + idx % MP];
     }
   };
-  auto stash = [&](int buf) {
+  auto stashA = [&](int buf) {
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
-      const int idx = tid + u * 256;
-      if (idx < MP * KC) As[buf][idx % KC][idx / KC] = ra[u];
-    }
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      const int idx = tid + u * 256;
-      Bs[buf][idx >> 6][idx & 63] = rb[u];
+      const int idx = tid + u * NT;
+      if (FULL || idx < MP * KC) As[buf][idx / MP][idx % MP] = ra[u];
     }
   };
-
-  const int nchunks = (K + KC - 1) / KC;
-  fetch(0);
-  stash(0);
-  __syncthreads();
-  for (int ch = 0; ch < nchunks; ++ch) {
-    const int cur = ch & 1;
-    if (ch + 1 < nchunks) fetch((ch + 1) * KC);
+  auto fetchB = [&](int slot, int ch) {
 #pragma unroll
-    for (int kk = 0; kk < KC / 4; ++kk) {
-      const int kr = kk * 4 + lk;
-      double b[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = Bs[cur][kr][c * 16 + li];
-#pragma unroll
-      for (int i = 0; i < MTW; ++i) {
-        const int t = wave + 4 * i;
-        if (t < MT) {
-          const double a = As[cur][kr][t * 16 + li];
-#pragma unroll
-          for (int c = 0; c < 4; ++c)
-            acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[c], acc[i][c], 0, 0, 0);
-        }
+    for (int s = 0; s < KS; ++s) {
+      const int kk = ch * KC + 4 * s + lk;
+      const int kb = min(kk, K - 1);
+      bq[slot][s] = B[(size_t)kb * ld + col];
+      lq[slot][s] = At[(size_t)kk * lda + MP];
+      if constexpr (UPD) {
+        wq[slot][s] = J.W[(size_t)kb * ld + col];
+        gq[slot][s] = J.g[kb];
       }
     }
-    if (ch + 1 < nchunks) stash(cur ^ 1);
-    __syncthreads();
+  };
+
+  fetchA(0);
+  stashA(0);
+#pragma unroll
+  for (int p = 0; p < R - 1; ++p) fetchB(p, min(p, nchunks - 1));
+  __syncthreads();
+
+  for (int ch0 = 0; ch0 < nchunks; ch0 += R) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int ch = ch0 + j;
+      if (ch < nchunks) {
+        const int cur = ch & 1;
+        fetchA(min(ch + 1, nchunks - 1));
+        fetchB((j + R - 1) % R, min(ch + R - 1, nchunks - 1));
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+          const int kr = s * 4 + lk;
+          double bv = bq[j][s];
+          if constexpr (UPD) {
+            // SBP substep update of the input: th - c * w * g, boundary rows zero
+            const int kk = ch * KC + kr;
+            bv = (kk == 0 || kk >= K - 1) ? 0.0 : bv - J.bc * wq[j][s] * gq[j][s];
+          }
+#pragma unroll
+          for (int t = 0; t < MT; ++t) {
+            const double a = As[cur][kr][t * 16 + li];
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
+          }
+          last = fma(lq[j][s], bv, last);
+        }
+        stashA(cur ^ 1);
+        __syncthreads();
+      }
+    }
   }
 
   // v_mfma_f64_16x16x4_f64 output: D[i][j] of a tile sits in lane
   // j + 16*(i%4), register i/4 (one row group per register).
+  const int mode = col >> 1;
 #pragma unroll
-  for (int i = 0; i < MTW; ++i) {
-    const int t = wave + 4 * i;
-    if (t >= MT) continue;
+  for (int t = 0; t < MT; ++t) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int col = n0 + c * 16 + li;
-      const int mode = col >> 1;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = t * 16 + r * 4 + lk;
-        if (row < M) {
-          const size_t idx = (size_t)row * ld + col;
-          C[idx] = epi_apply(e, acc[i][c][r], idx, mode);
-        }
+    for (int r = 0; r < 4; ++r) {
+      const int row = t * 16 + r * 4 + lk;
+      if (row < M) {
+        const size_t idx = (size_t)row * ld + col;
+        J.C[idx] = epi_apply(J.e, acc[t][r], idx, mode);
       }
+    }
+  }
+  if (M > MP) {
+    // the four k-residue partial sums sit in lanes li, li+16, li+32, li+48
+    last += __shfl_xor(last, 16);
+    last += __shfl_xor(last, 32);
+    if (lk == 0) {
+      const size_t idx = (size_t)MP * ld + col;
+      J.C[idx] = epi_apply(J.e, last, idx, mode);
     }
   }
 }
 
+int gemm_row_tiles(int M) { return (M > 256 ? 256 : M) / 16 + ((M > 256 ? 256 : M) % 16 != 0); }
+
+int gemm_op_lda(int mt) { return mt * 16 + 8; }
+
+int gemm_op_kp(int K) { return (K + GK_KC - 1) / GK_KC * GK_KC; }
+
+void gemm_pack_op(const double* A, int M, int K, int mt, double* out) {
+  const int lda = gemm_op_lda(mt), kp = gemm_op_kp(K), mp = mt * 16;
+  for (size_t i = 0; i < (size_t)lda * kp; ++i) out[i] = 0.0;
+  for (int r = 0; r < M && r <= mp; ++r)
+    for (int k = 0; k < K; ++k) out[(size_t)k * lda + r] = A[(size_t)r * K + k];
+}
+
 template <int MT>
-static hipError_t launch_mt(int M, int N, int K, const double* A, int lda, const double* B,
-                            double* C, int ld, const Epi& e, hipStream_t s) {
-  dim3 grid(N / 64);
-  hipLaunchKernelGGL(gemm_f64_kernel<MT>, grid, dim3(256), 0, s, M, K, A, lda, B, C, ld, e);
+static hipError_t launch_mt(const GemmBatch& b, int nj, bool upd, hipStream_t s) {
+  dim3 grid(b.ld / (GK_NW * 16), nj);
+  if (upd)
+    hipLaunchKernelGGL((gemm_f64_kernel<MT, true>), grid, dim3(GK_NW * 64), 0, s, b);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<MT, false>), grid, dim3(GK_NW * 64), 0, s, b);
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_f64(int M, int N, int K, const double* A, int lda, const double* B,
-                           double* C, int ld, const Epi& e, hipStream_t s) {
-  if (N % 64 != 0 || M <= 0 || K <= 0) return hipErrorInvalidValue;
-  const int mt = (M + 15) / 16;
+hipError_t launch_gemm_batch(const GemmBatch& b, int nj, hipStream_t s) {
+  if (nj < 1 || nj > GEMM_MAX_JOBS || b.ld % (GK_NW * 16) != 0) return hipErrorInvalidValue;
+  const int mt = b.job[0].op.mt;
+  bool upd = false;
+  for (int j = 0; j < nj; ++j) {
+    const GemmOp& op = b.job[j].op;
+    if (op.mt != mt || op.M < 1 || op.M > mt * 16 + 1 || op.K < 1 || op.kp % GK_KC != 0 ||
+        op.kp < op.K || op.lda != gemm_op_lda(mt) || !op.At || !b.job[j].B || !b.job[j].C)
+      return hipErrorInvalidValue;
+    const bool u = b.job[j].W != nullptr;
+    if (j > 0 && u != upd) return hipErrorInvalidValue;
+    if (u && !b.job[j].g) return hipErrorInvalidValue;
+    upd = u;
+  }
   switch (mt) {
-#define NHQG_MT(n) case n: return launch_mt<n>(M, N, K, A, lda, B, C, ld, e, s);
+#define NHQG_MT(n) case n: return launch_mt<n>(b, nj, upd, s);
     NHQG_MT(1) NHQG_MT(2) NHQG_MT(3) NHQG_MT(4) NHQG_MT(5) NHQG_MT(6) NHQG_MT(7) NHQG_MT(8)
     NHQG_MT(9) NHQG_MT(10) NHQG_MT(11) NHQG_MT(12) NHQG_MT(13) NHQG_MT(14) NHQG_MT(15)
-    NHQG_MT(16) NHQG_MT(17)
+    NHQG_MT(16)
 #undef NHQG_MT
     default: return hipErrorInvalidValue;
   }

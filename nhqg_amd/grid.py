@@ -13,7 +13,7 @@ Supported branch (anything else raises): float64, ``horizontal_dealiasing=
 ``sbp_transfer_mode='interp'``, ``thermal_closure`` either
 ``'fixed_conduction'`` or ``'evolve_mean'`` with
 ``mean_exchange_discretization='balanced_sbp2_pc'``, either ``w_bc_top``,
-power-of-two ``Nx`` in [16, 1024], ``Nz`` <= 271.
+power-of-two ``Nx`` in [16, 1024], ``Nz`` <= 256.
 """
 
 from __future__ import annotations
@@ -344,8 +344,8 @@ def _check_config(cfg: NHQGConfig) -> None:
         raise NotImplementedError("vertical_cutoff_n is not built")
     if cfg.Nx < 16 or cfg.Nx > 1024 or cfg.Nx & (cfg.Nx - 1):
         raise NotImplementedError("Nx must be a power of two in [16, 1024]")
-    if cfg.Nz < 3 or cfg.Nz > 271:
-        raise NotImplementedError("Nz must be in [3, 271]")
+    if cfg.Nz < 3 or cfg.Nz > 256:
+        raise NotImplementedError("Nz must be in [3, 256]")
 
 
 def make_grid(cfg: NHQGConfig, shells: str = "all") -> Grid:

@@ -185,8 +185,11 @@ class DeviceModel:
                 LU[i, k] = f
                 for j in range(k + 1, min(k + 3, n)):
                     LU[i, j] -= f * LU[k, j]
-        for _ in range(ns):
-            ths = o["T2S"] @ th
+        # the substeps stay on the SBP grid: one cycle operator T2S @ S2T
+        # per substep, S2T only after the last one
+        P = o["T2S"] @ o["S2T"]
+        ths = o["T2S"] @ th
+        for s in range(ns):
             fl = self.parseval(wsbp, ths)
             tb = o["thb2s"] @ thb
             rhs = B @ tb - mu * sdt * (D1 @ fl)
@@ -206,8 +209,8 @@ class DeviceModel:
             ths = ths - sdt * wsbp * gh[:, None]
             ths[0] = 0.0
             ths[-1] = 0.0
-            th = o["S2T"] @ ths
-        return th, thb
+            ths = (P if s + 1 < ns else o["S2T"]) @ ths
+        return ths, thb
 
     def sanitize(self, f):
         t = self.t
