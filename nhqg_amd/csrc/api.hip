@@ -298,7 +298,8 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
 
 // matrix stream + rhs, R_th in + w, theta out
 double shell_bytes(const nhqg_ctx* c) {
-  return 8.0 * c->P.n_shells * (double)c->Ng * c->Ng + 4.0 * 16.0 * c->Ng * (double)c->nm;
+  const double np = shell_pad_order(c->Ng);
+  return 8.0 * c->P.n_shells * np * np + 4.0 * 16.0 * c->Ng * (double)c->nm;
 }
 
 // imex_implicit_solve (nhqg/solver.py:691-738), q_boundary='none'.
@@ -619,7 +620,20 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(upload(c, (int**)&o.pairs, h->herm_pairs, (size_t)2 * p->n_pairs));
   RUN(upload(c, &o.shell_ptr, h->shell_ptr, (size_t)p->n_shells + 1));
   RUN(upload(c, &o.shell_modes, h->shell_modes, (size_t)nm));
-  RUN(upload(c, &o.shell_inv, h->shell_inv, (size_t)p->n_shells * Ng * Ng));
+  {
+    // shell inverses, padded for the MFMA solve; packed in slabs of shells
+    const int np = shell_pad_order(Ng);
+    const size_t per = (size_t)np * np;
+    RUN(dalloc(c, &o.shell_inv, (size_t)p->n_shells * per));
+    const int slab = 128;
+    std::vector<double> buf((size_t)slab * per);
+    for (int s0 = 0; s0 < p->n_shells; s0 += slab) {
+      const int s1 = std::min(p->n_shells, s0 + slab);
+      shell_pack_inv(h->shell_inv, p->n_shells, Ng, s0, s1, buf.data());
+      CK(hipMemcpy(o.shell_inv + (size_t)s0 * per, buf.data(), (size_t)(s1 - s0) * per * 8,
+                   hipMemcpyHostToDevice));
+    }
+  }
   RUN(upload(c, &o.eta_phys, h->eta_phys, (size_t)c->Nx * c->Nx));
   RUN(upload(c, &o.sponge_phys, h->sponge_phys, (size_t)c->Nx * c->Nx));
   RUN(upload(c, (double**)&o.tw, h->twiddle, (size_t)2 * c->Nx));

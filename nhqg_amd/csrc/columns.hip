@@ -8,66 +8,123 @@
 // Shell solve (nhqg/solver.py:648-675, 720-736): for every active |k|^2
 // shell s, w = inv_s * rhs for all modes of the shell, fused with the theta
 // back-substitution theta = (R_th + alpha * w) * inv_alpha_th.
-// invT holds each shell inverse transposed ([s][j][i]) so that the threads
-// of a block (one output row each) read consecutive addresses.  The kernel
-// is bound by streaming the (Nz-1)^2 matrices once per call.
+//
+// One workgroup per shell, 8 modes (16 real columns = one MFMA tile width)
+// per pass: W (np x 16) = inv_s (np x np) * X (np x 16) on
+// v_mfma_f64_16x16x4_f64, wave w owning row tiles 4w .. 4w+3 of the
+// 64*TW-padded order np.  The shell matrices (packed by shell_pack_inv:
+// inv_s[j][i] at s*np*np + j*np + i, zero padded) stream from HBM straight
+// into the A operand through a register ring SH_PD k-steps deep; X is
+// gathered into LDS once per pass.  The kernel is bound by streaming the
+// matrices once per call (~np^2 * 8 bytes per shell).
 // ---------------------------------------------------------------------------
 #define SH_CH 8
+#define SH_PD 8
 
+template <int TW>
 __global__ void __launch_bounds__(256)
 shell_solve_kernel(int Ng, int ld, const int* __restrict__ shell_ptr,
-                   const int* __restrict__ shell_modes, const double* __restrict__ invT,
+                   const int* __restrict__ shell_modes, const double* __restrict__ invP,
                    const double* __restrict__ rhs, double* __restrict__ w_out,
                    const double* __restrict__ Rth, double* __restrict__ th_out,
                    const double* __restrict__ iath, double alpha) {
-  extern __shared__ double rs[];  // [Ng][2*SH_CH]
+  constexpr int NP = 64 * TW;       // padded order; 4 waves x TW row tiles of 16
+  constexpr int KSTEPS = NP / 4;
+  static_assert(KSTEPS % SH_PD == 0, "prefetch depth must divide the k-steps");
+  __shared__ double xs[NP][16];
   __shared__ int ms[SH_CH];
-  const int s = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int p0 = shell_ptr[s], p1 = shell_ptr[s + 1];
-  const double* M = invT + (size_t)s * Ng * Ng;
+  const int sh = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int p0 = shell_ptr[sh], p1 = shell_ptr[sh + 1];
+  // A fragment of row tile t at k-step k: inv[t*16 + li][4k + lk]
+  const double* Ms = invP + (size_t)sh * NP * NP + (size_t)lk * NP + wave * TW * 16 + li;
   for (int c0 = p0; c0 < p1; c0 += SH_CH) {
     const int nc = min(SH_CH, p1 - c0);
     if (tid < SH_CH) ms[tid] = tid < nc ? shell_modes[c0 + tid] : -1;
     __syncthreads();
-    for (int idx = tid; idx < Ng * 2 * SH_CH; idx += 256) {
-      const int j = idx / (2 * SH_CH), c = idx % (2 * SH_CH);
+    for (int idx = tid; idx < NP * 16; idx += 256) {
+      const int j = idx >> 4, c = idx & 15;
       const int m = ms[c >> 1];
-      rs[idx] = m >= 0 ? rhs[(size_t)j * ld + 2 * m + (c & 1)] : 0.0;
+      xs[j][c] = (m >= 0 && j < Ng) ? rhs[(size_t)j * ld + 2 * m + (c & 1)] : 0.0;
     }
     __syncthreads();
-    for (int i = tid; i < Ng; i += 256) {
-      double acc[2 * SH_CH];
+    v4d acc[TW];
 #pragma unroll
-      for (int c = 0; c < 2 * SH_CH; ++c) acc[c] = 0.0;
-#pragma unroll 4
-      for (int j = 0; j < Ng; ++j) {
-        const double a = M[(size_t)j * Ng + i];
-        const double* r = rs + j * 2 * SH_CH;
+    for (int t = 0; t < TW; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    double a[SH_PD][TW];
 #pragma unroll
-        for (int c = 0; c < 2 * SH_CH; ++c) acc[c] = fma(a, r[c], acc[c]);
+    for (int p = 0; p < SH_PD; ++p)
+#pragma unroll
+      for (int t = 0; t < TW; ++t) a[p][t] = Ms[(size_t)4 * p * NP + t * 16];
+#pragma unroll 1
+    for (int k0 = 0; k0 < KSTEPS; k0 += SH_PD) {
+#pragma unroll
+      for (int p = 0; p < SH_PD; ++p) {
+        const int k = k0 + p;
+        const double b = xs[4 * k + lk][li];
+#pragma unroll
+        for (int t = 0; t < TW; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p][t], b, acc[t], 0, 0, 0);
+        const int kn = min(k + SH_PD, KSTEPS - 1);
+#pragma unroll
+        for (int t = 0; t < TW; ++t) a[p][t] = Ms[(size_t)4 * kn * NP + t * 16];
+        // compiler-only fence: keeps each refill SH_PD k-steps ahead of its
+        // use (otherwise the loads are re-formed at the top of the consuming
+        // iteration and their latency is exposed)
+        asm volatile("" ::: "memory");
       }
+    }
+    // D[i][n] of a tile sits in lane n + 16*(i%4), register i/4
+    const int m = ms[li >> 1];
+    if (m >= 0) {
+      const double ia = iath[m];
 #pragma unroll
-      for (int c = 0; c < 2 * SH_CH; ++c) {
-        const int m = ms[c >> 1];
-        if (m >= 0) {
-          const size_t idx = (size_t)i * ld + 2 * m + (c & 1);
-          w_out[idx] = acc[c];
-          th_out[idx] = (Rth[idx] + alpha * acc[c]) * iath[m];
+      for (int t = 0; t < TW; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = (wave * TW + t) * 16 + r * 4 + lk;
+          if (row < Ng) {
+            const size_t idx = (size_t)row * ld + 2 * m + (li & 1);
+            const double w = acc[t][r];
+            w_out[idx] = w;
+            th_out[idx] = (Rth[idx] + alpha * w) * ia;
+          }
         }
-      }
     }
     __syncthreads();
   }
 }
 
+int shell_pad_order(int Ng) { return 64 * ((Ng + 63) / 64); }
+
+void shell_pack_inv(const double* invT, int n_shells, int Ng, int s0, int s1, double* out) {
+  const int np = shell_pad_order(Ng);
+  for (size_t i = 0; i < (size_t)(s1 - s0) * np * np; ++i) out[i] = 0.0;
+  for (int s = s0; s < s1; ++s) {
+    const double* src = invT + (size_t)s * Ng * Ng;
+    double* dst = out + (size_t)(s - s0) * np * np;
+    for (int j = 0; j < Ng; ++j)
+      for (int i = 0; i < Ng; ++i) dst[(size_t)j * np + i] = src[(size_t)j * Ng + i];
+  }
+  (void)n_shells;
+}
+
 hipError_t launch_shell_solve(int n_shells, int Ng, int ld, const int* shell_ptr,
-                              const int* shell_modes, const double* invT, const double* rhs,
+                              const int* shell_modes, const double* invP, const double* rhs,
                               double* w_out, const double* Rth, double* th_out,
                               const double* iath, double alpha, hipStream_t s) {
-  const size_t shm = (size_t)Ng * 2 * SH_CH * sizeof(double);
-  hipLaunchKernelGGL(shell_solve_kernel, dim3(n_shells), dim3(256), shm, s, Ng, ld, shell_ptr,
-                     shell_modes, invT, rhs, w_out, Rth, th_out, iath, alpha);
+  const int tw = shell_pad_order(Ng) / 64;
+  switch (tw) {
+#define NHQG_TW(n)                                                                          \
+  case n:                                                                                   \
+    hipLaunchKernelGGL(shell_solve_kernel<n>, dim3(n_shells), dim3(256), 0, s, Ng, ld,      \
+                       shell_ptr, shell_modes, invP, rhs, w_out, Rth, th_out, iath, alpha); \
+    break;
+    NHQG_TW(1) NHQG_TW(2) NHQG_TW(3) NHQG_TW(4)
+#undef NHQG_TW
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -5,7 +5,7 @@
 // column 2m / 2m+1 hold Re / Im of mode m.  Modes are ordered ky-major,
 // m = ky_i * nkx + kx_i, over the kept rows (|kx| <= Nx/3 for the 2/3 band,
 // or all rows) and kept ky columns; nm_pad rounds the mode count up to a
-// multiple of 64 so every contraction tile is full.  Padding modes carry
+// multiple of 64 so every contraction workgroup (128 columns) is full.  Padding modes carry
 // zeros and zero per-mode factors.
 #pragma once
 
@@ -92,8 +92,13 @@ struct FluxArgs {
 // pass: 0 all three passes, 1 col_inverse, 2 row products, 3 col_forward
 hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s);
 hipError_t launch_speed(const FluxArgs& a, double* rowmax, hipStream_t s);
+// Shell matrices for launch_shell_solve: order np = 64*ceil(Ng/64), shell s
+// at s*np*np, inv_s[j][i] at j*np + i (the transposed inverse), zero padded.
+int shell_pad_order(int Ng);
+// packs shells [s0, s1) of invT (n_shells x Ng x Ng, transposed) into out
+void shell_pack_inv(const double* invT, int n_shells, int Ng, int s0, int s1, double* out);
 hipError_t launch_shell_solve(int n_shells, int Ng, int ld, const int* shell_ptr,
-                              const int* shell_modes, const double* invT, const double* rhs,
+                              const int* shell_modes, const double* invP, const double* rhs,
                               double* w_out, const double* Rth, double* th_out,
                               const double* iath, double alpha, hipStream_t s);
 hipError_t launch_parseval(int nl, int nm, int ld, int P, const double* a, const double* b,
@@ -149,7 +154,7 @@ int gemm_op_lda(int mt);
 int gemm_op_kp(int K);
 void gemm_pack_op(const double* A, int M, int K, int mt, double* out);
 // Runs b.job[0..nj) in one launch; every job has the same mt and all or
-// none use the update.  ld must be a multiple of 64.
+// none use the update.  ld must be a multiple of 128.
 hipError_t launch_gemm_batch(const GemmBatch& b, int nj, hipStream_t s);
 // out[l][c] = epilogue(0) over nl rows of ld doubles.
 hipError_t launch_combine(int nl, int ld, double* out, const Epi& e, hipStream_t s);

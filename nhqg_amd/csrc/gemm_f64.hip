@@ -21,9 +21,15 @@
 // for the A stash leaves the ring in flight.
 #include "common.h"
 
+#ifndef GK_KC
 #define GK_KC 8   // k-chunk staged in LDS per iteration (double buffered)
-#define GK_NW 4   // waves per workgroup; 16 columns each
-#define GK_R 3    // B ring depth in chunks
+#endif
+#ifndef GK_NW
+#define GK_NW 8   // waves per workgroup; 16 columns each
+#endif
+#ifndef GK_R
+#define GK_R 2    // B ring depth in chunks
+#endif
 
 static_assert(GK_KC % 4 == 0, "k-chunk must be a multiple of the MFMA depth");
 
