@@ -32,32 +32,72 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 
 
-def roofline(prof: dict, g, lay) -> dict:
-    """Dominant kernel class of one step and its achieved rate."""
-    name, (ms, n, work) = max(prof.items(), key=lambda kv: kv[1][0])
-    out = {"kernel": name, "launches_per_step": n, "ms_per_step": ms}
-    nq, nm = g.Nz + 1, lay.nm
-    if name == "gemm_f64_mfma":
-        ach = work / (ms * 1e-3) / 1e12
-        out.update(bound="mfma", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                   frac=ach / PEAK_FP64_TFLOPS,
-                   work="4*M*K flops per complex mode (real operator x complex field), "
-                        f"{nm} modes of the 2/3 band per launch")
-    elif name.startswith("flux"):
-        # algorithmic bytes of the whole FFT-product pipeline: read q, w, theta
-        # nodal + write three tendencies, per RHS (band-compact modes)
-        t = sum(v[0] for k, v in prof.items() if k.startswith("flux"))
-        nrhs = prof["flux_col_inverse"][1]
-        alg = 6.0 * 16.0 * nq * nm * nrhs
-        ach = alg / (t * 1e-3) / 1e9
-        out.update(kernel="flux pipeline (3 passes)", ms_per_step=t, bound="hbm", achieved=ach,
-                   peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
-                   work="6 fields x (Nz+1) levels x band modes x 16 B per RHS")
-    else:
-        ach = work / (ms * 1e-3) / 1e9
-        out.update(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s",
-                   frac=ach / PEAK_HBM_GBS, work="bytes moved per launch")
-    out["traffic"] = None
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_polar_256x256x256.md")
+# kernel-class name (nhqg_profile_step) -> HIP kernels it launches
+CLASS_KERNELS = {
+    "gemm_f64_mfma": ("gemm_f64_kernel",),
+    "flux": ("col_inverse_flux", "row_flux", "col_forward_flux"),
+    "shell_solve": ("shell_solve_kernel",),
+    "parseval_means": ("parseval_partial_kernel",),
+}
+
+
+def profiled_traffic(cls: str):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (256^3 workload only): dispatch-weighted
+    mean over the class's kernels, or None when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            lines = [ln.split() for ln in f if ln.strip().startswith("traffic ")]
+    except OSError:
+        return None
+    n = tot = 0.0
+    for _, name, cnt, fb, wb in lines:
+        if name.startswith(CLASS_KERNELS[cls]) and not name.startswith("col_inverse_flux<256,false>"):
+            n += float(cnt)
+            tot += float(cnt) * (float(fb) + float(wb))
+    return tot / n if n else None
+
+
+def rooflines(prof: dict, g, lay, full_size: bool) -> dict:
+    """Achieved rate of each kernel class in one profiled step, against the
+    MI355X roofline that bounds it.  Work is algorithmic, in the units a
+    launch processes: 4*M*K flops per complex band mode for a contraction;
+    16 B per complex band mode per field-level moved for the HBM-bound
+    classes (inputs read once, outputs written once)."""
+    nq, ng, nm = g.Nz + 1, g.Nz - 1, lay.nm
+    out = {}
+    ms, n, work = prof["gemm_f64_mfma"]
+    ach = work / (ms * 1e-3) / 1e12
+    out["gemm_f64_mfma"] = dict(
+        kernel="gemm_f64_kernel (batched vertical contractions)", launches_per_step=n,
+        ms_per_step=ms, avg_launch_us=1e3 * ms / n, bound="mfma", achieved=ach,
+        peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP64_TFLOPS,
+        work_per_launch=work / n,
+        work="4*M*K flops per complex mode of the 2/3 band (real operator x complex field), "
+             f"{nm} modes per contraction, 1-3 contractions per launch",
+        traffic=profiled_traffic("gemm_f64_mfma") if full_size else None)
+    t = sum(prof[k][0] for k in ("flux_col_inverse", "flux_row_products", "flux_col_forward"))
+    nrhs = prof["flux_col_inverse"][1]
+    alg = 6.0 * 16.0 * nq * nm * nrhs          # q, w, theta nodal in + 3 tendencies out
+    ach = alg / (t * 1e-3) / 1e9
+    out["flux"] = dict(
+        kernel="col_inverse_flux + row_flux + col_forward_flux (one FFT-product pipeline per RHS)",
+        launches_per_step=3 * nrhs, ms_per_step=t, avg_launch_us=1e3 * t / (3 * nrhs), bound="hbm",
+        achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
+        work_per_launch=alg / (3 * nrhs),
+        work="6 fields (q, w, theta nodal in; 3 tendencies out) x (Nz+1) levels x band modes x 16 B "
+             "per RHS",
+        traffic=profiled_traffic("flux") if full_size else None)
+    ms, n, work = prof["shell_solve"]
+    ach = work / (ms * 1e-3) / 1e9
+    out["shell_solve"] = dict(
+        kernel="shell_solve_kernel", launches_per_step=n, ms_per_step=ms,
+        avg_launch_us=1e3 * ms / n, bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s",
+        frac=ach / PEAK_HBM_GBS, work_per_launch=work / n,
+        work="padded shell inverses streamed once (n_shells x 256^2 x 8 B) + rhs, R_th in, "
+             "w, theta out",
+        traffic=profiled_traffic("shell_solve") if full_size else None)
     return out
 
 
@@ -164,11 +204,20 @@ def main() -> int:
             "nx": g.Nx, "nz": g.Nz, "band_modes": lay.nm, "stored_modes": g.Nx * g.Nk,
             "parallelism": "1 GPU",
         },
-        "roofline": roofline(prof, g, lay),
+        "roofline": None,
         "hbm_copy_gbps": copy_gbps,
         "kernels": {k: {"ms_per_step": v[0], "launches": v[1]} for k, v in prof.items()},
+        "launches_per_step": int(sum(v[1] for v in prof.values())),
+        "device_bytes": int(sp.ctx.device_bytes()),
         "diagnostics_after": {k: diag[k] for k in ("KE_tot", "max_speed", "Nusselt_dealiased")},
     }
+    rl = rooflines(prof, g, lay, full_size=(args.nz == 256))
+    dom = max(rl, key=lambda k: rl[k]["ms_per_step"])
+    res["roofline"] = dict(rl[dom], kernel_class=dom)
+    res["rooflines"] = rl
+    res["roofline_note"] = ("traffic: HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/"
+                            "WRITE_SIZE summary (" + os.path.relpath(TRAFFIC_FILE, ROOT) + "), "
+                            "calibrated on a 1 GiB copy")
     if not args.no_cpu:
         base, par = cpu_baseline(cfg, st0, args.cpu_steps, one)
         res["cpu_baseline"] = base
