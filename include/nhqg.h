@@ -99,6 +99,13 @@ typedef struct {
   const double *eta_phys;     /* trap background PV on the grid */
   const double *sponge_phys;  /* band-limited sponge rate */
   const double *twiddle;      /* 2*nx: exp(-2 pi i k / nx) */
+  /* Shell factors built on the device (used when shell_inv is NULL):
+   * A_s = coef[s][0] I - coef[s][1] K - coef[s][2] B0, inverted in FP64 by
+   * Gauss-Jordan with partial pivoting (replaces the host inverse loop of
+   * _build_imex_inv, nhqg/grid.py:349-432, for q_boundary='none'). */
+  const double *shell_B0;     /* Ng x Ng: w_pinv G_Z G_Z w_stencil */
+  const double *shell_K;      /* Ng x Ng buoyancy map (Neumann top) or NULL (K = I) */
+  const double *shell_coef;   /* n_shells x 3: alpha, beta, c */
 } nhqg_operators;
 
 /* Diagnostics vector layout returned by nhqg_diagnostics (length
@@ -174,6 +181,10 @@ const char *nhqg_timing_name(int32_t i);
  * context stream; returns the achieved GB/s (read + write bytes).  Used to
  * calibrate HBM traffic counters and as a practical bandwidth ceiling. */
 int nhqg_stream_copy(nhqg_ctx *ctx, uint64_t bytes, int32_t reps, double *gbps);
+
+/* Copy the inverse of the s-th shell factor held by the context
+ * (imex_inv[s] of nhqg/grid.py:430, Ng x Ng row-major) to out. */
+int nhqg_get_shell_inverse(nhqg_ctx *ctx, int32_t s, double *out);
 
 /* Bytes of device memory held by the context. */
 int nhqg_device_bytes(nhqg_ctx *ctx, uint64_t *bytes);

@@ -47,7 +47,8 @@ _OPS_FIELDS = [
     ("inv_alpha_q", _dp), ("inv_alpha_th", _dp), ("diss_q", _dp), ("diss_w", _dp),
     ("diss_th", _dp), ("mask", _dp), ("pweight", _dp), ("partner", _ip),
     ("herm_pairs", _ip), ("shell_ptr", _ip), ("shell_modes", _ip), ("shell_inv", _dp),
-    ("eta_phys", _dp), ("sponge_phys", _dp), ("twiddle", _dp),
+    ("eta_phys", _dp), ("sponge_phys", _dp), ("twiddle", _dp), ("shell_B0", _dp),
+    ("shell_K", _dp), ("shell_coef", _dp),
 ]
 
 
@@ -59,6 +60,7 @@ EXPORTS = (
     "nhqg_create", "nhqg_destroy", "nhqg_set_state", "nhqg_get_state", "nhqg_step",
     "nhqg_step_async", "nhqg_sync", "nhqg_diagnostics", "nhqg_apply", "nhqg_get_aux",
     "nhqg_last_step_ms", "nhqg_profile_step", "nhqg_timing_name", "nhqg_device_bytes",
+    "nhqg_get_shell_inverse",
     "nhqg_last_error", "nhqg_stream_copy",
 )
 
@@ -96,6 +98,7 @@ def load():
     lib.nhqg_timing_name.argtypes = [C.c_int32]
     lib.nhqg_timing_name.restype = C.c_char_p
     lib.nhqg_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+    lib.nhqg_get_shell_inverse.argtypes = [vp, C.c_int32, _dp]
     lib.nhqg_stream_copy.argtypes = [vp, C.c_uint64, C.c_int32, _dp]
     lib.nhqg_last_error.argtypes = [vp]
     lib.nhqg_last_error.restype = C.c_char_p
@@ -254,6 +257,14 @@ class Context:
         self._check(self.lib.nhqg_stream_copy(self.h, int(nbytes), int(reps), C.byref(v)),
                     "nhqg_stream_copy")
         return v.value
+
+    def shell_inverse(self, s: int) -> np.ndarray:
+        """inv(A'(k)) of the s-th touched shell as held on the device."""
+        n = self.grid.Nz - 1
+        out = np.empty((n, n))
+        self._check(self.lib.nhqg_get_shell_inverse(self.h, int(s), _ptr(out)),
+                    "nhqg_get_shell_inverse")
+        return out
 
     def device_bytes(self):
         v = C.c_uint64()

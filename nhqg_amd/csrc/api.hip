@@ -621,17 +621,47 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(upload(c, &o.shell_ptr, h->shell_ptr, (size_t)p->n_shells + 1));
   RUN(upload(c, &o.shell_modes, h->shell_modes, (size_t)nm));
   {
-    // shell inverses, padded for the MFMA solve; packed in slabs of shells
+    // shell inverses, padded for the MFMA solve: uploaded (packed in slabs
+    // of shells) or built and inverted on the device
     const int np = shell_pad_order(Ng);
     const size_t per = (size_t)np * np;
     RUN(dalloc(c, &o.shell_inv, (size_t)p->n_shells * per));
-    const int slab = 128;
-    std::vector<double> buf((size_t)slab * per);
-    for (int s0 = 0; s0 < p->n_shells; s0 += slab) {
-      const int s1 = std::min(p->n_shells, s0 + slab);
-      shell_pack_inv(h->shell_inv, p->n_shells, Ng, s0, s1, buf.data());
-      CK(hipMemcpy(o.shell_inv + (size_t)s0 * per, buf.data(), (size_t)(s1 - s0) * per * 8,
-                   hipMemcpyHostToDevice));
+    if (h->shell_inv) {
+      const int slab = 128;
+      std::vector<double> buf((size_t)slab * per);
+      for (int s0 = 0; s0 < p->n_shells; s0 += slab) {
+        const int s1 = std::min(p->n_shells, s0 + slab);
+        shell_pack_inv(h->shell_inv, p->n_shells, Ng, s0, s1, buf.data());
+        CK(hipMemcpy(o.shell_inv + (size_t)s0 * per, buf.data(), (size_t)(s1 - s0) * per * 8,
+                     hipMemcpyHostToDevice));
+      }
+    } else {
+      if (!h->shell_B0 || !h->shell_coef) {
+        c->err = "shell factors need shell_inv or shell_B0 + shell_coef";
+        return NHQG_ERR_ARG;
+      }
+      std::vector<double> t((size_t)Ng * Ng);
+      double *B0T = nullptr, *KT = nullptr, *coef = nullptr;
+      int* status = nullptr;
+      for (int i = 0; i < Ng; ++i)
+        for (int j = 0; j < Ng; ++j) t[(size_t)j * Ng + i] = h->shell_B0[(size_t)i * Ng + j];
+      RUN(upload(c, &B0T, t.data(), t.size()));
+      if (h->shell_K) {
+        for (int i = 0; i < Ng; ++i)
+          for (int j = 0; j < Ng; ++j) t[(size_t)j * Ng + i] = h->shell_K[(size_t)i * Ng + j];
+        RUN(upload(c, &KT, t.data(), t.size()));
+      }
+      RUN(upload(c, &coef, h->shell_coef, (size_t)3 * p->n_shells));
+      RUN(dalloc(c, &status, 1));
+      CK(hipMemset(status, 0, sizeof(int)));
+      LK(launch_shell_build(p->n_shells, Ng, B0T, KT, coef, o.shell_inv, status, c->st));
+      int st = 0;
+      CK(hipStreamSynchronize(c->st));
+      CK(hipMemcpy(&st, status, sizeof(int), hipMemcpyDeviceToHost));
+      if (st) {
+        c->err = "singular shell factor (zero pivot)";
+        return NHQG_ERR_ARG;
+      }
     }
   }
   RUN(upload(c, &o.eta_phys, h->eta_phys, (size_t)c->Nx * c->Nx));
@@ -1070,6 +1100,19 @@ int nhqg_stream_copy(nhqg_ctx* c, uint64_t bytes, int32_t reps, double* gbps) {
   (void)hipFree(a);
   (void)hipFree(b);
   return r;
+}
+
+int nhqg_get_shell_inverse(nhqg_ctx* c, int32_t s, double* out) {
+  if (!c || !out || s < 0 || s >= c->P.n_shells) return NHQG_ERR_ARG;
+  const int Ng = c->Ng, np = shell_pad_order(Ng);
+  std::vector<double> t((size_t)np * np);
+  CK(hipStreamSynchronize(c->st));
+  CK(hipMemcpy(t.data(), c->o.shell_inv + (size_t)s * np * np, t.size() * sizeof(double),
+               hipMemcpyDeviceToHost));
+  // the slot holds inv_s[j][i] at j*np + i
+  for (int i = 0; i < Ng; ++i)
+    for (int j = 0; j < Ng; ++j) out[(size_t)i * Ng + j] = t[(size_t)j * np + i];
+  return NHQG_OK;
 }
 
 int nhqg_device_bytes(nhqg_ctx* c, uint64_t* bytes) {

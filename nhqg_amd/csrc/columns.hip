@@ -96,6 +96,120 @@ shell_solve_kernel(int Ng, int ld, const int* __restrict__ shell_ptr,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Shell factors on the device (nhqg/grid.py:349-432, q_boundary='none'):
+// one workgroup per shell writes M = A_s^T = (alpha I - beta K - c B0)^T
+// into its padded slot (zeros outside Ng x Ng) and inverts it in place by
+// Gauss-Jordan elimination with partial pivoting in FP64.  Row swaps of M
+// are undone as column swaps at the end, so the slot ends up holding
+// (A_s^-1)^T, the layout shell_solve_kernel reads.  B0T and KT are the
+// transposed matrices (coalesced reads); a zero pivot sets *status.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024)
+shell_build_kernel(int Ng, int np, const double* __restrict__ B0T, const double* __restrict__ KT,
+                   const double* __restrict__ coef, double* __restrict__ invP,
+                   int* __restrict__ status) {
+  __shared__ double rowk[256], colk[256];
+  __shared__ int piv[256];
+  __shared__ double wv[16];
+  __shared__ int wi[16];
+  const int sh = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  double* M = invP + (size_t)sh * np * np;
+  const double al = coef[3 * sh], be = coef[3 * sh + 1], cc = coef[3 * sh + 2];
+  // M[j][i] = A[i][j] = alpha d_ij - beta K[i][j] - c B0[i][j]
+  for (int j = wave; j < np; j += 16)
+    for (int i = lane; i < np; i += 64) {
+      double v = 0.0;
+      if (i < Ng && j < Ng) {
+        double a = (i == j) ? al : 0.0;
+        if (KT) a -= be * KT[(size_t)j * Ng + i];
+        v = a - cc * B0T[(size_t)j * Ng + i];
+      }
+      M[(size_t)j * np + i] = v;
+    }
+  __syncthreads();
+  for (int k = 0; k < Ng; ++k) {
+    // pivot: largest |M[r][k]|, r >= k (lowest index on ties)
+    double best = -1.0;
+    int bi = k;
+    for (int r = k + tid; r < Ng; r += 1024) {
+      const double v = fabs(M[(size_t)r * np + k]);
+      if (v > best) {
+        best = v;
+        bi = r;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      wv[wave] = best;
+      wi[wave] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double b = wv[0];
+      int i0 = wi[0];
+      for (int w = 1; w < 16; ++w)
+        if (wv[w] > b || (wv[w] == b && wi[w] < i0)) {
+          b = wv[w];
+          i0 = wi[w];
+        }
+      piv[k] = i0;
+      if (!(b > 0.0)) *status = 1;
+    }
+    __syncthreads();
+    const int p = piv[k];
+    if (p != k)
+      for (int c = tid; c < Ng; c += 1024) {
+        const double t = M[(size_t)k * np + c];
+        M[(size_t)k * np + c] = M[(size_t)p * np + c];
+        M[(size_t)p * np + c] = t;
+      }
+    __syncthreads();
+    const double pv = M[(size_t)k * np + k];
+    for (int c = tid; c < Ng; c += 1024)
+      rowk[c] = (c == k) ? 1.0 / pv : M[(size_t)k * np + c] / pv;
+    for (int r = tid; r < Ng; r += 1024) colk[r] = (r == k) ? 0.0 : M[(size_t)r * np + k];
+    __syncthreads();
+    for (int r = wave; r < Ng; r += 16) {
+      const double f = colk[r];
+      for (int c = lane; c < Ng; c += 64) {
+        double* e = M + (size_t)r * np + c;
+        if (r == k)
+          *e = rowk[c];
+        else
+          *e = ((c == k) ? 0.0 : *e) - f * rowk[c];
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = Ng - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k)
+      for (int r = tid; r < Ng; r += 1024) {
+        const double t = M[(size_t)r * np + k];
+        M[(size_t)r * np + k] = M[(size_t)r * np + p];
+        M[(size_t)r * np + p] = t;
+      }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_shell_build(int n_shells, int Ng, const double* B0T, const double* KT,
+                              const double* coef, double* invP, int* status, hipStream_t s) {
+  if (Ng > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(shell_build_kernel, dim3(n_shells), dim3(1024), 0, s, Ng,
+                     shell_pad_order(Ng), B0T, KT, coef, invP, status);
+  return hipGetLastError();
+}
+
 int shell_pad_order(int Ng) { return 64 * ((Ng + 63) / 64); }
 
 void shell_pack_inv(const double* invT, int n_shells, int Ng, int s0, int s1, double* out) {

@@ -97,6 +97,10 @@ hipError_t launch_speed(const FluxArgs& a, double* rowmax, hipStream_t s);
 int shell_pad_order(int Ng);
 // packs shells [s0, s1) of invT (n_shells x Ng x Ng, transposed) into out
 void shell_pack_inv(const double* invT, int n_shells, int Ng, int s0, int s1, double* out);
+// builds and inverts every shell factor on the device into invP (layout
+// above); *status is set nonzero on a zero pivot
+hipError_t launch_shell_build(int n_shells, int Ng, const double* B0T, const double* KT,
+                              const double* coef, double* invP, int* status, hipStream_t s);
 hipError_t launch_shell_solve(int n_shells, int Ng, int ld, const int* shell_ptr,
                               const int* shell_modes, const double* invP, const double* rhs,
                               double* w_out, const double* Rth, double* th_out,

@@ -181,11 +181,12 @@ def _invert_stack(A: np.ndarray) -> np.ndarray:
     return out
 
 
-def shell_inverses(ksq_vals, cfg, gamma_imex, B0, Kb, Ld_inv_sq) -> np.ndarray:
-    """inv(A'(k)) for each listed shell value, A' in w's Galerkin basis with
-    buoyancy eliminated and the scalar q stage (q_boundary='none')."""
+def shell_coefficients(ksq_vals, cfg, gamma_imex, Ld_inv_sq, neumann: bool):
+    """Per-shell scalars of A'(k) = alpha I - beta K - c B0 (w's Galerkin
+    basis, buoyancy eliminated, scalar q stage for q_boundary='none';
+    nhqg/grid.py:395-429).  For the Dirichlet top K = I is folded into
+    alpha and beta = 0."""
     g = gamma_imex * cfg.dt
-    Ng = B0.shape[0]
     kp = ksq_vals ** cfg.hyper_order
     a_q = 1.0 + g * (cfg.nu_q * kp + cfg.drag)
     a_w = 1.0 + g * cfg.nu_w * kp
@@ -193,6 +194,16 @@ def shell_inverses(ksq_vals, cfg, gamma_imex, B0, Kb, Ld_inv_sq) -> np.ndarray:
     Ra_s = cfg.Ra_tilde / cfg.sigma
     den = ksq_vals + Ld_inv_sq
     ck = np.where(den == 0.0, 0.0, 1.0 / np.where(den == 0.0, 1.0, den))
+    c = g * g * ck / a_q
+    if neumann:
+        return a_w, g * g * Ra_s / a_t, c
+    return a_w - g * g * Ra_s / a_t, np.zeros_like(a_w), c
+
+
+def shell_inverses(ksq_vals, cfg, gamma_imex, B0, Kb, Ld_inv_sq) -> np.ndarray:
+    """inv(A'(k)) for each listed shell value on the host (numpy/LAPACK)."""
+    alpha, beta, c = shell_coefficients(ksq_vals, cfg, gamma_imex, Ld_inv_sq, Kb is not None)
+    Ng = B0.shape[0]
     I = np.eye(Ng)
     out = np.empty((ksq_vals.size, Ng, Ng))
     step = 512
@@ -200,10 +211,10 @@ def shell_inverses(ksq_vals, cfg, gamma_imex, B0, Kb, Ld_inv_sq) -> np.ndarray:
         e = min(ksq_vals.size, s + step)
         sl = slice(s, e)
         if Kb is None:
-            couple = (a_w[sl] - g * g * Ra_s / a_t[sl])[:, None, None] * I[None]
+            couple = alpha[sl][:, None, None] * I[None]
         else:
-            couple = a_w[sl][:, None, None] * I[None] - (g * g * Ra_s / a_t[sl])[:, None, None] * Kb[None]
-        A = couple - (g * g * ck[sl] / a_q[sl])[:, None, None] * B0[None]
+            couple = alpha[sl][:, None, None] * I[None] - beta[sl][:, None, None] * Kb[None]
+        A = couple - c[sl][:, None, None] * B0[None]
         out[sl] = _invert_stack(A)
     return out
 
@@ -279,6 +290,9 @@ class Grid:
     sbp_transfer_mode: str
     sbp_corrector_substeps: int
     buoyancy_K: np.ndarray | None = None
+    # 'device': shell factors built and inverted on the GPU (FP64);
+    # 'host': numpy/LAPACK inverses uploaded (used to cross-check)
+    shell_build: str = "device"
     _layouts: dict = field(default_factory=dict, repr=False)
     _contexts: dict = field(default_factory=dict, repr=False)
 
@@ -578,9 +592,19 @@ def device_operators(g: Grid, lay: ModeLayout) -> dict:
     ops["sponge_phys"] = g.sponge_phys
     k = np.arange(Nx)
     ops["twiddle"] = np.stack([np.cos(2.0 * np.pi * k / Nx), -np.sin(2.0 * np.pi * k / Nx)], 1)
-    # shell inverses of the touched shells, transposed for coalesced reads
+    # shell factors of the touched shells: built and inverted on the device
+    # in FP64 from B0, K and three scalars per shell (default), or inverted
+    # here with LAPACK and uploaded transposed
     vals = g._shell_values[lay.tables["shell_ids"]]
-    inv = shell_inverses(vals, g.cfg, g.gamma_imex, shell_matrix_B0(g), g.buoyancy_K,
-                         g.Ld_inv_sq)
-    ops["shell_inv"] = np.ascontiguousarray(inv.transpose(0, 2, 1))
+    B0 = shell_matrix_B0(g)
+    if g.shell_build == "host":
+        inv = shell_inverses(vals, g.cfg, g.gamma_imex, B0, g.buoyancy_K, g.Ld_inv_sq)
+        ops["shell_inv"] = np.ascontiguousarray(inv.transpose(0, 2, 1))
+    else:
+        alpha, beta, c = shell_coefficients(vals, g.cfg, g.gamma_imex, g.Ld_inv_sq,
+                                            g.buoyancy_K is not None)
+        ops["shell_inv"] = None
+        ops["shell_B0"] = B0
+        ops["shell_K"] = g.buoyancy_K
+        ops["shell_coef"] = np.stack([alpha, beta, c], axis=1)
     return ops

@@ -21,6 +21,16 @@ class DeviceModel:
         self.t = layout.tables
         self.o = device_operators(grid, layout)
         self.nm = layout.nm
+        if self.o["shell_inv"] is None:
+            # device-built factors: A_s = alpha I - beta K - c B0 (transposed
+            # inverses, as the device holds them)
+            B0, K, coef = self.o["shell_B0"], self.o["shell_K"], self.o["shell_coef"]
+            I = np.eye(B0.shape[0])
+            inv = []
+            for al, be, c in coef:
+                A = al * I - c * B0 if K is None else al * I - be * K - c * B0
+                inv.append(np.linalg.inv(A).T)
+            self.o["shell_inv"] = np.array(inv)
 
     # ---- layout ----
     def to_compact(self, f):

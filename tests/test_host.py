@@ -43,11 +43,22 @@ def test_operators_match_oracle(cfgf):
     if o.sponge_phys is not None:
         assert rel(g.sponge_phys, o.sponge_phys) < 1e-14
     lay = g.layout(False)
-    ops = device_operators(g, lay)
     ids = lay.tables["shell_ids"]
-    inv = ops["shell_inv"].transpose(0, 2, 1)
+    g.shell_build = "host"
+    inv = device_operators(g, lay)["shell_inv"].transpose(0, 2, 1)
     for k in range(0, ids.size, max(1, ids.size // 7)):
         assert rel(inv[k], o.imex_inv[ids[k]]) < 1e-13
+    # the device recipe: A_s = alpha I - beta K - c B0 from three scalars
+    g.shell_build = "device"
+    ops = device_operators(g, lay)
+    assert ops["shell_inv"] is None
+    B0, K, coef = ops["shell_B0"], ops["shell_K"], ops["shell_coef"]
+    I = np.eye(B0.shape[0])
+    for k in range(0, ids.size, max(1, ids.size // 7)):
+        A = coef[k, 0] * I - coef[k, 2] * B0
+        if K is not None:
+            A = coef[k, 0] * I - coef[k, 1] * K - coef[k, 2] * B0
+        assert rel(np.linalg.inv(A), o.imex_inv[ids[k]]) < 1e-13
 
 
 def test_layout_tables():
