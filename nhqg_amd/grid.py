@@ -227,8 +227,10 @@ def shell_inverses(ksq_vals, cfg, gamma_imex, B0, Kb, Ld_inv_sq) -> np.ndarray:
 @dataclass
 class Grid:
     """Precomputed operators; attribute names follow the reference's Grid
-    (nhqg/grid.py:29-141).  ``imex_inv`` holds the shells listed in
-    ``shell_ids`` (the ones the device layout touches)."""
+    (nhqg/grid.py:29-141).  The per-shell implicit factors (the reference's
+    ``imex_inv``) are not held here: a device context builds and inverts
+    the shells its mode layout touches, on the GPU in FP64
+    (``shell_build='device'``) or from host LAPACK inverses ('host')."""
 
     cfg: NHQGConfig
     Nx: int
@@ -365,9 +367,9 @@ def _check_config(cfg: NHQGConfig) -> None:
 def make_grid(cfg: NHQGConfig, shells: str = "all") -> Grid:
     """Build all host operators for ``cfg`` (nhqg/grid.py:439-789).
 
-    shells: 'all' inverts every |k|^2 shell like the reference; 'band'
-    inverts only the shells with a mode inside the 2/3 band (all the device
-    step touches for band-limited states)."""
+    shells: accepted for call compatibility ('all', 'band', 'none'); the
+    distinct |k|^2 shell values are always recorded and the factors of the
+    shells a context touches are built when the context is created."""
     _check_config(cfg)
     Nx, N, L = cfg.Nx, cfg.Nz, cfg.L
     Nk = Nx // 2 + 1
