@@ -32,6 +32,7 @@ extern "C" {
 #define NHQG_ERR_HIP 2         /* HIP runtime error (see nhqg_last_error) */
 #define NHQG_ERR_UNSUPPORTED 3 /* configuration branch not built */
 #define NHQG_ERR_STATE 4       /* call out of order (e.g. step before set_state) */
+#define NHQG_NONFINITE 5        /* nhqg_diagnostics: output written, some value non-finite */
 
 typedef struct nhqg_ctx nhqg_ctx;
 
@@ -145,7 +146,10 @@ int nhqg_step_async(nhqg_ctx *ctx, int32_t n_steps);
 int nhqg_sync(nhqg_ctx *ctx);
 
 /* Judged diagnostics of the current state (nhqg/diagnostics.py:597-675):
- * out[NHQG_DIAG_*] scalars then q, w, theta vertical spectra. */
+ * out[NHQG_DIAG_*] scalars then q, w, theta vertical spectra.  Returns
+ * NHQG_NONFINITE (output still written) when any value is NaN or infinite:
+ * the condition on which the reference driver saves a checkpoint and stops
+ * (scripts/run_polar.py:364-367). */
 int nhqg_diagnostics(nhqg_ctx *ctx, double *out, int32_t n_out);
 
 /* Component entry points on the device state, for parity testing:

@@ -16,6 +16,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnhqg.so
 _lib = None
 
 NHQG_OK = 0
+NHQG_NONFINITE = 5
 OP_EXPLICIT_RHS = 1
 OP_IMPLICIT_TEND = 2
 OP_IMPLICIT_SOLVE = 3
@@ -235,7 +236,9 @@ class Context:
     def diagnostics(self):
         n = N_DIAG_SCALARS + 3 * (self.grid.Nz + 1)
         out = np.zeros(n, np.float64)
-        self._check(self.lib.nhqg_diagnostics(self.h, _ptr(out), n), "nhqg_diagnostics")
+        rc = self.lib.nhqg_diagnostics(self.h, _ptr(out), n)
+        if rc != NHQG_NONFINITE:
+            self._check(rc, "nhqg_diagnostics")
         return out
 
     def profile_step(self):

@@ -865,6 +865,8 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
   out[NHQG_DIAG_Q_RMS] = std::sqrt(q2);
   out[NHQG_DIAG_W_RMS] = std::sqrt(w2);
   out[NHQG_DIAG_TH_RMS] = std::sqrt(t2);
+  for (int i = 0; i < NHQG_DIAG_SCALARS + 3 * Nq; ++i)
+    if (!std::isfinite(out[i])) return NHQG_NONFINITE;
   return NHQG_OK;
 }
 
