@@ -90,6 +90,8 @@ def rooflines(prof: dict, g, lay, full_size: bool) -> dict:
              "per RHS",
         traffic=profiled_traffic("flux") if full_size else None)
     ms, n, work = prof["shell_solve"]
+    if n == 0:
+        return out   # shell solves ran as contractions in the eigenbasis of B0
     ach = work / (ms * 1e-3) / 1e9
     out["shell_solve"] = dict(
         kernel="shell_solve_kernel", launches_per_step=n, ms_per_step=ms,

@@ -107,6 +107,14 @@ typedef struct {
   const double *shell_B0;     /* Ng x Ng: w_pinv G_Z G_Z w_stencil */
   const double *shell_K;      /* Ng x Ng buoyancy map (Neumann top) or NULL (K = I) */
   const double *shell_coef;   /* n_shells x 3: alpha, beta, c */
+  /* Dirichlet-top w: B0 = Q diag(lam) Q^-1 (real spectrum), so every shell
+   * solve is w = Q diag(1 / (alpha_m - c_m lam)) Q^-1 rhs.  When eig_Q is
+   * set (and nz w is Dirichlet) no shell matrices are built or streamed. */
+  const double *eig_Q;        /* Ng x Ng eigenvectors of B0 */
+  const double *eig_Qinv;     /* Ng x Ng */
+  const double *eig_lam;      /* Ng eigenvalues */
+  const double *mode_alpha;   /* per mode: alpha of its shell */
+  const double *mode_c;       /* per mode: c of its shell */
 } nhqg_operators;
 
 /* Diagnostics vector layout returned by nhqg_diagnostics (length

@@ -49,7 +49,8 @@ _OPS_FIELDS = [
     ("diss_th", _dp), ("mask", _dp), ("pweight", _dp), ("partner", _ip),
     ("herm_pairs", _ip), ("shell_ptr", _ip), ("shell_modes", _ip), ("shell_inv", _dp),
     ("eta_phys", _dp), ("sponge_phys", _dp), ("twiddle", _dp), ("shell_B0", _dp),
-    ("shell_K", _dp), ("shell_coef", _dp),
+    ("shell_K", _dp), ("shell_coef", _dp), ("eig_Q", _dp), ("eig_Qinv", _dp), ("eig_lam", _dp),
+    ("mode_alpha", _dp), ("mode_c", _dp),
 ]
 
 

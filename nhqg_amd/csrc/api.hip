@@ -23,6 +23,9 @@ struct Ops {
   // contraction operators, packed k-major (GemmOp); Psbp = T2S * S2T is one
   // full SBP substep cycle (SBP grid -> Galerkin theta -> SBP grid)
   GemmOp V, VSw, VSd, Vinv, Pw, Pd, Qw, GSw, W2S, T2S, S2T, Psbp, Mtw, Mwt, Mth2w, Sw, Sd, row_cc;
+  // Dirichlet-top shell solves in the eigenbasis of B0 (eigQ.At set)
+  GemmOp eigQ, eigQinv;
+  double *lam, *mal, *mc, *dtab;
   double *thb2s, *s2thb, *proj_d;
   double *d1b, *lb, *cc, *cc_sbp;
   int *rows, *kidx_of_x, *full_idx, *partner, *shell_ptr, *shell_modes;
@@ -195,6 +198,8 @@ struct Job {
   const double* W = nullptr;
   const double* g = nullptr;
   double bc = 0.0;
+  double* C2 = nullptr;
+  Epi e2 = epi_make(0.0);
 };
 
 int gemm_jobs(nhqg_ctx* c, std::initializer_list<Job> jobs) {
@@ -216,6 +221,8 @@ int gemm_jobs(nhqg_ctx* c, std::initializer_list<Job> jobs) {
     g.g = j.g;
     g.bc = j.bc;
     g.e = j.e;
+    g.C2 = j.C2;
+    g.e2 = j.e2;
     work += 4.0 * j.op->M * j.op->K * (double)c->nm;
   }
   PL(CL_GEMM, work, launch_gemm_batch(b, n, c->st));
@@ -314,9 +321,23 @@ int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double
     epi_add(e, 1.0, Rw);
     epi_add(e, ars, Rth, o.iath);
     RUN(gemm(c, o.Qw, Rq, c->rhs_w, e));
-    PL(CL_SHELL, shell_bytes(c),
-       launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes, o.shell_inv,
-                          c->rhs_w, ow, Rth, oth, o.iath, al, c->st));
+    if (o.eigQ.At) {
+      // w = Q diag(1 / (alpha_m - c_m lam)) Q^-1 rhs_w, theta = (R_th + alpha w) iath
+      Epi d = epi_make(1.0);
+      d.d_tab = o.dtab;
+      d.d_ld = c->nm_pad;
+      RUN(gemm(c, o.eigQinv, c->rhs_w, c->tmpg, d));
+      Job j{&o.eigQ, c->tmpg, ow, epi_make(1.0)};
+      j.C2 = oth;
+      j.e2 = epi_make(al);
+      epi_add(j.e2, 1.0, Rth);
+      j.e2.s_out = o.iath;
+      RUN(gemm_jobs(c, {j}));
+    } else {
+      PL(CL_SHELL, shell_bytes(c),
+         launch_shell_solve(c->P.n_shells, c->Ng, c->ld, o.shell_ptr, o.shell_modes,
+                            o.shell_inv, c->rhs_w, ow, Rth, oth, o.iath, al, c->st));
+    }
   } else {
     RUN(gemm(c, o.Mth2w, Rth, c->tmpg, epi_make(1.0)));
     Epi e = epi_make(al);
@@ -620,9 +641,28 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(upload(c, (int**)&o.pairs, h->herm_pairs, (size_t)2 * p->n_pairs));
   RUN(upload(c, &o.shell_ptr, h->shell_ptr, (size_t)p->n_shells + 1));
   RUN(upload(c, &o.shell_modes, h->shell_modes, (size_t)nm));
-  {
+  const bool eigen = h->eig_Q && h->eig_Qinv && h->eig_lam && h->mode_alpha && h->mode_c &&
+                     !p->w_neumann;
+  if (eigen) {
+    RUN(upload_op(c, &o.eigQ, h->eig_Q, Ng, Ng, mt));
+    RUN(upload_op(c, &o.eigQinv, h->eig_Qinv, Ng, Ng, mt));
+    RUN(upload(c, &o.lam, h->eig_lam, (size_t)Ng));
+    // padding modes: alpha 1, c 0 (their columns are zero)
+    std::vector<double> a((size_t)c->nm_pad, 1.0), cm((size_t)c->nm_pad, 0.0);
+    std::memcpy(a.data(), h->mode_alpha, sizeof(double) * nm);
+    std::memcpy(cm.data(), h->mode_c, sizeof(double) * nm);
+    RUN(upload(c, &o.mal, a.data(), a.size()));
+    RUN(upload(c, &o.mc, cm.data(), cm.size()));
+    RUN(dalloc(c, &o.dtab, (size_t)Ng * c->nm_pad));
+    LK(launch_eigen_diag(Ng, c->nm_pad, o.mal, o.mc, o.lam, o.dtab, c->st));
+    CK(hipStreamSynchronize(c->st));
+    o.shell_inv = nullptr;
+  } else {
     // shell inverses, padded for the MFMA solve: uploaded (packed in slabs
     // of shells) or built and inverted on the device
+    o.eigQ = GemmOp{nullptr, 0, 0, 0, 0, 0};
+    o.eigQinv = o.eigQ;
+    o.lam = o.mal = o.mc = o.dtab = nullptr;
     const int np = shell_pad_order(Ng);
     const size_t per = (size_t)np * np;
     RUN(dalloc(c, &o.shell_inv, (size_t)p->n_shells * per));
@@ -1106,6 +1146,10 @@ int nhqg_stream_copy(nhqg_ctx* c, uint64_t bytes, int32_t reps, double* gbps) {
 
 int nhqg_get_shell_inverse(nhqg_ctx* c, int32_t s, double* out) {
   if (!c || !out || s < 0 || s >= c->P.n_shells) return NHQG_ERR_ARG;
+  if (!c->o.shell_inv) {
+    c->err = "shell solves run in the eigenbasis of B0: no per-shell matrices";
+    return NHQG_ERR_UNSUPPORTED;
+  }
   const int Ng = c->Ng, np = shell_pad_order(Ng);
   std::vector<double> t((size_t)np * np);
   CK(hipStreamSynchronize(c->st));

@@ -210,6 +210,25 @@ hipError_t launch_shell_build(int n_shells, int Ng, const double* B0T, const dou
   return hipGetLastError();
 }
 
+// Diagonal of the shell solves in the eigenbasis of B0, built once per
+// context: tab[r][m] = 1 / (alpha_m - c_m lam_r).
+__global__ void __launch_bounds__(256)
+eigen_diag_kernel(int nl, int nmp, const double* __restrict__ alpha, const double* __restrict__ c,
+                  const double* __restrict__ lam, double* __restrict__ tab) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)nl * nmp) return;
+  const int r = (int)(idx / nmp), m = (int)(idx % nmp);
+  tab[idx] = 1.0 / (alpha[m] - c[m] * lam[r]);
+}
+
+hipError_t launch_eigen_diag(int nl, int nmp, const double* alpha, const double* c,
+                             const double* lam, double* tab, hipStream_t s) {
+  const size_t n = (size_t)nl * nmp;
+  hipLaunchKernelGGL(eigen_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nl,
+                     nmp, alpha, c, lam, tab);
+  return hipGetLastError();
+}
+
 int shell_pad_order(int Ng) { return 64 * ((Ng + 63) / 64); }
 
 void shell_pack_inv(const double* invT, int n_shells, int Ng, int s0, int s1, double* out) {
@@ -303,6 +322,7 @@ hipError_t launch_parseval(int nl, int nm, int ld, int P, const double* a, const
 // Laplacian as [i*3 + (j-i+1)].
 // ---------------------------------------------------------------------------
 #define SBP_NMAX 272
+#define SBP_MV_ROWS 8
 
 // y[i] = sum_j M[i][j] x[j] for a small dense row-major M, one wavefront per
 // row: lanes stride the row (coalesced) and a fixed xor-shuffle tree sums
@@ -315,13 +335,31 @@ NHQG_DEV double wave_sum(double v) {
 
 NHQG_DEV void block_matvec(int n_out, int n_in, const double* __restrict__ M, const double* x,
                            double* y) {
+  // SBP_MV_ROWS rows per wavefront at a time, so that the row loads of a
+  // round are all in flight together (the matrices come from L2 / HBM)
+  constexpr int R = SBP_MV_ROWS;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int i = wave; i < n_out; i += nw) {
-    const double* r = M + (size_t)i * n_in;
-    double t = 0.0;
-    for (int j = lane; j < n_in; j += 64) t += r[j] * x[j];
-    t = wave_sum(t);
-    if (lane == 0) y[i] = t;
+  for (int i0 = wave * R; i0 < n_out; i0 += nw * R) {
+    double t[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) t[r] = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < (SBP_NMAX + 63) / 64; ++jj) {
+      const int j = lane + 64 * jj;
+      if (j < n_in) {
+        const double xj = x[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = min(i0 + r, n_out - 1);
+          t[r] += M[(size_t)i * n_in + j] * xj;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double v = wave_sum(t[r]);
+      if (lane == 0 && i0 + r < n_out) y[i0 + r] = v;
+    }
   }
 }
 
@@ -332,28 +370,40 @@ NHQG_DEV double band3(const double* b3, int i, int j, int n) {
   return b3[i * 3 + o + 1];
 }
 
-__global__ void __launch_bounds__(256)
+// Corrector preamble (nhqg/solver.py:930-942) in one block: <w_sbp^2> per
+// level from the Parseval partials, the band matrices B = I + half and
+// A = I - half (boundary rows of A set to identity), the pentadiagonal LU of
+// A without pivoting (one thread, register recurrence) and the explicit
+// inverse A^-1 (one column per thread; the forward-substitution vector is
+// kept in the output and read back by the same thread).
+__global__ void __launch_bounds__(512)
 sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
                    const double* __restrict__ lb, double c1, double c2, double* __restrict__ Bg,
-                   double* __restrict__ LUg) {
+                   double* __restrict__ ainv) {
   __shared__ double w2[SBP_NMAX];
-  __shared__ double LU[SBP_NMAX * 5];
+  __shared__ double d1s[3 * SBP_NMAX], lbs[3 * SBP_NMAX];
+  __shared__ double LU[SBP_NMAX * 5];   // per row: a (-2), b (-1), d, e (+1), f (+2)
+  __shared__ double rd[SBP_NMAX];       // 1 / pivot
   const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += 256) {
-    double s = 0.0;
-    for (int p = 0; p < P; ++p) s += partial[(size_t)i * P + p];
-    w2[i] = s;
+  for (int i = tid; i < 3 * n; i += 512) {
+    d1s[i] = d1b[i];
+    lbs[i] = lb[i];
+  }
+  for (int i = tid; i < n; i += 512) {
+    double sum = 0.0;
+    for (int p = 0; p < P; ++p) sum += partial[(size_t)i * P + p];
+    w2[i] = sum;
   }
   __syncthreads();
-  for (int i = tid; i < n; i += 256) {
+  for (int i = tid; i < n; i += 512) {
     for (int o = -2; o <= 2; ++o) {
       const int j = i + o;
       double h = 0.0;
       if (j >= 0 && j < n) {
-        h = c1 * band3(lb, i, j, n);
+        h = c1 * band3(lbs, i, j, n);
         for (int k = i - 1; k <= i + 1; ++k) {
           if (k < 0 || k >= n) continue;
-          h += c2 * band3(d1b, i, k, n) * w2[k] * band3(d1b, k, j, n);
+          h += c2 * band3(d1s, i, k, n) * w2[k] * band3(d1s, k, j, n);
         }
       }
       const double id = (o == 0) ? 1.0 : 0.0;
@@ -363,50 +413,69 @@ sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const doubl
   }
   __syncthreads();
   if (tid == 0) {
+    // row k+1 (b1, d1, e1) and row k+2 (a2, b2, d2) carried in registers
+    double d0 = LU[2], e0 = LU[3], f0 = LU[4];
+    double b1 = LU[5 + 1], d1 = LU[5 + 2], e1 = LU[5 + 3];
+    double a2 = n > 2 ? LU[10 + 0] : 0.0, b2 = n > 2 ? LU[10 + 1] : 0.0;
+    double d2 = n > 2 ? LU[10 + 2] : 0.0;
     for (int k = 0; k < n - 1; ++k) {
-      const double piv = LU[k * 5 + 2];
-      for (int i = k + 1; i <= min(k + 2, n - 1); ++i) {
-        const double f = LU[i * 5 + (k - i) + 2] / piv;
-        LU[i * 5 + (k - i) + 2] = f;
-        for (int j = k + 1; j <= min(k + 2, n - 1); ++j)
-          LU[i * 5 + (j - i) + 2] -= f * LU[k * 5 + (j - k) + 2];
+      const double r = 1.0 / d0;
+      rd[k] = r;
+      const double m1 = b1 * r;
+      LU[(k + 1) * 5 + 1] = m1;
+      const double nd1 = d1 - m1 * e0;
+      const double ne1 = e1 - m1 * f0;
+      double na2 = 0.0, nb2 = 0.0, nd2 = 0.0;
+      if (k + 2 < n) {
+        const double m2 = a2 * r;
+        LU[(k + 2) * 5 + 0] = m2;
+        nb2 = b2 - m2 * e0;
+        nd2 = d2 - m2 * f0;
       }
+      LU[(k + 1) * 5 + 2] = nd1;
+      LU[(k + 1) * 5 + 3] = ne1;
+      // shift the window: row k+1 becomes the pivot row
+      d0 = nd1;
+      e0 = ne1;
+      f0 = LU[(k + 1) * 5 + 4];
+      b1 = nb2;
+      d1 = nd2;
+      e1 = (k + 2 < n) ? LU[(k + 2) * 5 + 3] : 0.0;
+      if (k + 3 < n) {
+        na2 = LU[(k + 3) * 5 + 0];
+        a2 = na2;
+        b2 = LU[(k + 3) * 5 + 1];
+        d2 = LU[(k + 3) * 5 + 2];
+      }
+    }
+    rd[n - 1] = 1.0 / d0;
+  }
+  __syncthreads();
+  // column c of A^-1: L y = e_c (y_i = 0 for i < c), then U x = y
+  double* __restrict__ yv = ainv;
+  const double* __restrict__ yr = ainv;
+  for (int c = tid; c < n; c += 512) {
+    double y1 = 0.0, y2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      if (i > c) v -= LU[i * 5 + 1] * y1 + LU[i * 5 + 0] * y2;
+      yv[(size_t)i * n + c] = v;
+      y2 = y1;
+      y1 = v;
     }
   }
   __syncthreads();
-  for (int i = tid; i < 5 * n; i += 256) LUg[i] = LU[i];
-}
-
-// Columns of A^{-1} from the band LU: thread c solves L y = e_c, U x = y
-// with y kept in LDS; writes column c of the row-major A^{-1}.
-#define SBP_COLS 16
-__global__ void __launch_bounds__(SBP_COLS)
-sbp_inverse_kernel(int n, const double* __restrict__ LUg, double* __restrict__ ainv) {
-  __shared__ double LU[SBP_NMAX * 5];
-  __shared__ double y[SBP_COLS][SBP_NMAX];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < 5 * n; i += SBP_COLS) LU[i] = LUg[i];
-  __syncthreads();
-  const int c = blockIdx.x * SBP_COLS + tid;
-  if (c >= n) return;
-  double y1 = 0.0, y2 = 0.0;
-  for (int i = 0; i < n; ++i) {
-    double v = (i == c) ? 1.0 : 0.0;
-    if (i >= 1) v -= LU[i * 5 + 1] * y1;
-    if (i >= 2) v -= LU[i * 5 + 0] * y2;
-    y[tid][i] = v;
-    y2 = y1;
-    y1 = v;
-  }
-  double x1 = 0.0, x2 = 0.0;
-  for (int i = n - 1; i >= 0; --i) {
-    double v = y[tid][i];
-    if (i + 1 < n) v -= LU[i * 5 + 3] * x1;
-    if (i + 2 < n) v -= LU[i * 5 + 4] * x2;
-    v /= LU[i * 5 + 2];
-    ainv[(size_t)i * n + c] = v;
-    x2 = x1;
-    x1 = v;
+  for (int c = tid; c < n; c += 512) {
+    double x1 = 0.0, x2 = 0.0;
+    for (int i = n - 1; i >= 0; --i) {
+      double v = yr[(size_t)i * n + c];
+      if (i + 1 < n) v -= LU[i * 5 + 3] * x1;
+      if (i + 2 < n) v -= LU[i * 5 + 4] * x2;
+      v *= rd[i];
+      yv[(size_t)i * n + c] = v;
+      x2 = x1;
+      x1 = v;
+    }
   }
 }
 
@@ -461,19 +530,16 @@ sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const doubl
   block_matvec(n, n, s2thb, nw, thb_io);
 }
 
-// sbp buffer (doubles, n = Nz+1): [0,5n) B band, [5n,10n) LU band,
-// [16n, 16n + n*n) A^{-1} row-major.
+// sbp buffer (doubles, n = Nz+1): [0,5n) B band, [16n, 16n + n*n) A^{-1}
+// row-major.
 hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double* d1b,
                               const double* lb, double c1, double c2, double* sbp,
                               hipStream_t s) {
   if (n > SBP_NMAX) return hipErrorInvalidValue;
   double* Bg = sbp;
-  double* LUg = sbp + 5 * (size_t)n;
   double* ainv = sbp + 16 * (size_t)n;
-  hipLaunchKernelGGL(sbp_prepare_kernel, dim3(1), dim3(256), 0, s, n, P, partial, d1b, lb, c1, c2,
-                     Bg, LUg);
-  hipLaunchKernelGGL(sbp_inverse_kernel, dim3((n + SBP_COLS - 1) / SBP_COLS), dim3(SBP_COLS), 0, s,
-                     n, LUg, ainv);
+  hipLaunchKernelGGL(sbp_prepare_kernel, dim3(1), dim3(512), 0, s, n, P, partial, d1b, lb, c1, c2,
+                     Bg, ainv);
   return hipGetLastError();
 }
 
@@ -488,7 +554,6 @@ hipError_t launch_sbp_substep(int n, int P, const double* partial, const double*
   return hipGetLastError();
 }
 
-// theta_sbp <- Z0(theta_sbp - sdt * w_sbp * g[l]); rows 0 and n-1 zeroed.
 // y = M x for small dense M (n_out x n_in); one block.
 __global__ void __launch_bounds__(1024)
 small_matvec_kernel(int n_out, int n_in, const double* __restrict__ M, const double* __restrict__ x,

@@ -30,8 +30,11 @@ NHQG_DEV double2 cmul_i(double2 a, double s) { return make_double2(-a.y * s, a.x
 // Linear epilogue shared by the contraction and the elementwise kernels:
 //   out = a_acc * s_acc[mode] * acc + sum_t aX[t] * sX[t][mode] * X[t][idx]
 //   out *= s_out[mode]                       (when s_out is set)
+//   out *= d_tab[row * d_ld + mode]          (when d_tab is set)
 // A null per-mode pointer means a factor of one; X terms with null
-// pointers are skipped.  idx is the element index in the output field.
+// pointers are skipped.  idx is the element index in the output field,
+// row its level.  The last factor is the diagonal of a shell solve in the
+// eigenbasis of B0 (A_k = alpha_k I - c_k B0).
 #define NHQG_EPI_TERMS 8
 struct Epi {
   double a_acc;
@@ -41,9 +44,11 @@ struct Epi {
   double aX[NHQG_EPI_TERMS];
   const double* sX[NHQG_EPI_TERMS];
   const double* s_out;
+  const double* d_tab;
+  int d_ld;
 };
 
-NHQG_DEV double epi_apply(const Epi& e, double acc, size_t idx, int mode) {
+NHQG_DEV double epi_apply(const Epi& e, double acc, size_t idx, int row, int mode) {
   double o = e.a_acc * acc;
   if (e.s_acc) o *= e.s_acc[mode];
   for (int t = 0; t < e.nX; ++t) {
@@ -52,6 +57,7 @@ NHQG_DEV double epi_apply(const Epi& e, double acc, size_t idx, int mode) {
     o += v;
   }
   if (e.s_out) o *= e.s_out[mode];
+  if (e.d_tab) o *= e.d_tab[(size_t)row * e.d_ld + mode];
   return o;
 }
 
@@ -66,6 +72,8 @@ static inline Epi epi_make(double a_acc) {
     e.sX[t] = nullptr;
   }
   e.s_out = nullptr;
+  e.d_tab = nullptr;
+  e.d_ld = 0;
   return e;
 }
 
@@ -97,6 +105,9 @@ hipError_t launch_speed(const FluxArgs& a, double* rowmax, hipStream_t s);
 int shell_pad_order(int Ng);
 // packs shells [s0, s1) of invT (n_shells x Ng x Ng, transposed) into out
 void shell_pack_inv(const double* invT, int n_shells, int Ng, int s0, int s1, double* out);
+// tab[r][m] = 1 / (alpha[m] - c[m] lam[r]) for r < nl, m < nmp (row pitch nmp)
+hipError_t launch_eigen_diag(int nl, int nmp, const double* alpha, const double* c,
+                             const double* lam, double* tab, hipStream_t s);
 // builds and inverts every shell factor on the device into invP (layout
 // above); *status is set nonzero on a zero pivot
 hipError_t launch_shell_build(int n_shells, int Ng, const double* B0T, const double* KT,
@@ -147,6 +158,8 @@ struct GemmJob {
   const double* g;
   double bc;
   Epi e;
+  double* C2;   // optional second output, epilogue e2 of the same product
+  Epi e2;
 };
 #define GEMM_MAX_JOBS 3
 struct GemmBatch {

@@ -142,7 +142,8 @@ gemm_f64_kernel(GemmBatch bt) {
       const int row = t * 16 + r * 4 + lk;
       if (row < M) {
         const size_t idx = (size_t)row * ld + col;
-        J.C[idx] = epi_apply(J.e, acc[t][r], idx, mode);
+        J.C[idx] = epi_apply(J.e, acc[t][r], idx, row, mode);
+        if (J.C2) J.C2[idx] = epi_apply(J.e2, acc[t][r], idx, row, mode);
       }
     }
   }
@@ -152,7 +153,8 @@ gemm_f64_kernel(GemmBatch bt) {
     last += __shfl_xor(last, 32);
     if (lk == 0) {
       const size_t idx = (size_t)MP * ld + col;
-      J.C[idx] = epi_apply(J.e, last, idx, mode);
+      J.C[idx] = epi_apply(J.e, last, idx, MP, mode);
+      if (J.C2) J.C2[idx] = epi_apply(J.e2, last, idx, MP, mode);
     }
   }
 }
@@ -209,7 +211,7 @@ combine_kernel(size_t n, int ld, double* out, Epi e) {
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= n) return;
   const int mode = (int)((idx % (size_t)ld) >> 1);
-  out[idx] = epi_apply(e, 0.0, idx, mode);
+  out[idx] = epi_apply(e, 0.0, idx, (int)(idx / (size_t)ld), mode);
 }
 
 hipError_t launch_combine(int nl, int ld, double* out, const Epi& e, hipStream_t s) {

@@ -295,6 +295,10 @@ class Grid:
     # 'device': shell factors built and inverted on the GPU (FP64);
     # 'host': numpy/LAPACK inverses uploaded (used to cross-check)
     shell_build: str = "device"
+    # Dirichlet-top w: 'eigen' solves every shell in the eigenbasis of B0
+    # (two dense contractions, no per-shell matrices); 'matrices' streams
+    # the per-shell inverses like the reference
+    shell_solver: str = "eigen"
     _layouts: dict = field(default_factory=dict, repr=False)
     _contexts: dict = field(default_factory=dict, repr=False)
 
@@ -599,6 +603,26 @@ def device_operators(g: Grid, lay: ModeLayout) -> dict:
     # here with LAPACK and uploaded transposed
     vals = g._shell_values[lay.tables["shell_ids"]]
     B0 = shell_matrix_B0(g)
+    if g.shell_solver == "eigen" and g.buoyancy_K is None:
+        # Dirichlet top: A_k = alpha_k I - c_k B0 for every shell, and B0 has
+        # a real spectrum with well-conditioned eigenvectors, so the solve
+        # is Q diag(1 / (alpha_k - c_k lam)) Q^-1 with no per-shell matrix
+        lam, Q = np.linalg.eig(B0)
+        if np.abs(lam.imag).max() > 1e-12 * np.abs(lam.real).max() or np.iscomplexobj(Q) and \
+                np.abs(Q.imag).max() > 1e-12:
+            raise NotImplementedError("B0 has a complex spectrum; use shell_solver='matrices'")
+        lam, Q = lam.real.copy(), np.real(Q).copy()
+        alpha, _, c = shell_coefficients(vals, g.cfg, g.gamma_imex, g.Ld_inv_sq, False)
+        ptr, modes = lay.tables["shell_ptr"], lay.tables["shell_modes"]
+        ma = np.ones(lay.nm)
+        mc = np.zeros(lay.nm)
+        for s in range(vals.size):
+            ms = modes[ptr[s]:ptr[s + 1]]
+            ma[ms] = alpha[s]
+            mc[ms] = c[s]
+        ops.update(shell_inv=None, eig_Q=Q, eig_Qinv=np.linalg.inv(Q), eig_lam=lam,
+                   mode_alpha=ma, mode_c=mc)
+        return ops
     if g.shell_build == "host":
         inv = shell_inverses(vals, g.cfg, g.gamma_imex, B0, g.buoyancy_K, g.Ld_inv_sq)
         ops["shell_inv"] = np.ascontiguousarray(inv.transpose(0, 2, 1))

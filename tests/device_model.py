@@ -21,7 +21,17 @@ class DeviceModel:
         self.t = layout.tables
         self.o = device_operators(grid, layout)
         self.nm = layout.nm
-        if self.o["shell_inv"] is None:
+        if self.o.get("eig_Q") is not None:
+            # eigenbasis solve: one inverse per shell from its modes' alpha, c
+            Q, Qi, lam = self.o["eig_Q"], self.o["eig_Qinv"], self.o["eig_lam"]
+            ptr, modes = self.t["shell_ptr"], self.t["shell_modes"]
+            inv = []
+            for s in range(ptr.size - 1):
+                m = modes[ptr[s]]
+                d = 1.0 / (self.o["mode_alpha"][m] - self.o["mode_c"][m] * lam)
+                inv.append((Q @ np.diag(d) @ Qi).T)
+            self.o["shell_inv"] = np.array(inv)
+        elif self.o["shell_inv"] is None:
             # device-built factors: A_s = alpha I - beta K - c B0 (transposed
             # inverses, as the device holds them)
             B0, K, coef = self.o["shell_B0"], self.o["shell_K"], self.o["shell_coef"]
