@@ -741,7 +741,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(field(c, &c->spw, Ng));
   RUN(field(c, &c->spth, Ng));
   RUN(dalloc(c, &c->partial, (size_t)Nq * c->Pred + 64));
-  RUN(dalloc(c, &c->sbp, (size_t)Nq * 16 + (size_t)Nq * Nq));
+  RUN(dalloc(c, &c->sbp, (size_t)Nq * 16));
   RUN(dalloc(c, &c->g, (size_t)Nq));
   c->fstride = (size_t)Nq * c->Nx * c->nky;
   RUN(dalloc(c, &c->X5, c->fstride * 8));

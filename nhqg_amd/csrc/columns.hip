@@ -372,14 +372,13 @@ NHQG_DEV double band3(const double* b3, int i, int j, int n) {
 
 // Corrector preamble (nhqg/solver.py:930-942) in one block: <w_sbp^2> per
 // level from the Parseval partials, the band matrices B = I + half and
-// A = I - half (boundary rows of A set to identity), the pentadiagonal LU of
-// A without pivoting (one thread, register recurrence) and the explicit
-// inverse A^-1 (one column per thread; the forward-substitution vector is
-// kept in the output and read back by the same thread).
+// A = I - half (boundary rows of A set to identity) and the pentadiagonal
+// LU of A without pivoting (one thread, register recurrence).  LUg gets the
+// bands (a, b multipliers; d, e, f of U) and then the reciprocal pivots.
 __global__ void __launch_bounds__(512)
 sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
                    const double* __restrict__ lb, double c1, double c2, double* __restrict__ Bg,
-                   double* __restrict__ ainv) {
+                   double* __restrict__ LUg) {
   __shared__ double w2[SBP_NMAX];
   __shared__ double d1s[3 * SBP_NMAX], lbs[3 * SBP_NMAX];
   __shared__ double LU[SBP_NMAX * 5];   // per row: a (-2), b (-1), d, e (+1), f (+2)
@@ -418,80 +417,81 @@ sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const doubl
     double b1 = LU[5 + 1], d1 = LU[5 + 2], e1 = LU[5 + 3];
     double a2 = n > 2 ? LU[10 + 0] : 0.0, b2 = n > 2 ? LU[10 + 1] : 0.0;
     double d2 = n > 2 ? LU[10 + 2] : 0.0;
-    for (int k = 0; k < n - 1; ++k) {
-      const double r = 1.0 / d0;
-      rd[k] = r;
-      const double m1 = b1 * r;
-      LU[(k + 1) * 5 + 1] = m1;
-      const double nd1 = d1 - m1 * e0;
-      const double ne1 = e1 - m1 * f0;
-      double na2 = 0.0, nb2 = 0.0, nd2 = 0.0;
-      if (k + 2 < n) {
-        const double m2 = a2 * r;
-        LU[(k + 2) * 5 + 0] = m2;
-        nb2 = b2 - m2 * e0;
-        nd2 = d2 - m2 * f0;
+    // chunks of 8 steps: the original band entries a step consumes (f of
+    // row k+1, e of row k+2, a, b, d of row k+3; none is stored before that
+    // step) are loaded before the chunk's dependent chain
+    constexpr int C = 8;
+    for (int k0 = 0; k0 < n - 1; k0 += C) {
+      double fk1[C], ek2[C], ak3[C], bk3[C], dk3[C];
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const int k = k0 + u;
+        const int r1 = min(k + 1, n - 1), r2 = min(k + 2, n - 1), r3 = min(k + 3, n - 1);
+        fk1[u] = LU[r1 * 5 + 4];
+        ek2[u] = LU[r2 * 5 + 3];
+        ak3[u] = LU[r3 * 5 + 0];
+        bk3[u] = LU[r3 * 5 + 1];
+        dk3[u] = LU[r3 * 5 + 2];
       }
-      LU[(k + 1) * 5 + 2] = nd1;
-      LU[(k + 1) * 5 + 3] = ne1;
-      // shift the window: row k+1 becomes the pivot row
-      d0 = nd1;
-      e0 = ne1;
-      f0 = LU[(k + 1) * 5 + 4];
-      b1 = nb2;
-      d1 = nd2;
-      e1 = (k + 2 < n) ? LU[(k + 2) * 5 + 3] : 0.0;
-      if (k + 3 < n) {
-        na2 = LU[(k + 3) * 5 + 0];
-        a2 = na2;
-        b2 = LU[(k + 3) * 5 + 1];
-        d2 = LU[(k + 3) * 5 + 2];
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const int k = k0 + u;
+        if (k < n - 1) {
+          const double r = 1.0 / d0;
+          rd[k] = r;
+          const double m1 = b1 * r;
+          LU[(k + 1) * 5 + 1] = m1;
+          const double nd1 = d1 - m1 * e0;
+          const double ne1 = e1 - m1 * f0;
+          double nb2 = 0.0, nd2 = 0.0;
+          if (k + 2 < n) {
+            const double m2 = a2 * r;
+            LU[(k + 2) * 5 + 0] = m2;
+            nb2 = b2 - m2 * e0;
+            nd2 = d2 - m2 * f0;
+          }
+          LU[(k + 1) * 5 + 2] = nd1;
+          LU[(k + 1) * 5 + 3] = ne1;
+          // shift the window: row k+1 becomes the pivot row
+          d0 = nd1;
+          e0 = ne1;
+          f0 = fk1[u];
+          b1 = nb2;
+          d1 = nd2;
+          e1 = (k + 2 < n) ? ek2[u] : 0.0;
+          if (k + 3 < n) {
+            a2 = ak3[u];
+            b2 = bk3[u];
+            d2 = dk3[u];
+          }
+        }
       }
     }
     rd[n - 1] = 1.0 / d0;
   }
   __syncthreads();
-  // column c of A^-1: L y = e_c (y_i = 0 for i < c), then U x = y
-  double* __restrict__ yv = ainv;
-  const double* __restrict__ yr = ainv;
-  for (int c = tid; c < n; c += 512) {
-    double y1 = 0.0, y2 = 0.0;
-    for (int i = 0; i < n; ++i) {
-      double v = (i == c) ? 1.0 : 0.0;
-      if (i > c) v -= LU[i * 5 + 1] * y1 + LU[i * 5 + 0] * y2;
-      yv[(size_t)i * n + c] = v;
-      y2 = y1;
-      y1 = v;
-    }
-  }
-  __syncthreads();
-  for (int c = tid; c < n; c += 512) {
-    double x1 = 0.0, x2 = 0.0;
-    for (int i = n - 1; i >= 0; --i) {
-      double v = yr[(size_t)i * n + c];
-      if (i + 1 < n) v -= LU[i * 5 + 3] * x1;
-      if (i + 2 < n) v -= LU[i * 5 + 4] * x2;
-      v *= rd[i];
-      yv[(size_t)i * n + c] = v;
-      x2 = x1;
-      x1 = v;
-    }
-  }
+  for (int i = tid; i < 5 * n; i += 512) LUg[i] = LU[i];
+  for (int i = tid; i < n; i += 512) LUg[5 * n + i] = rd[i];
 }
 
-// One substep's 1-D solve: flux -> rhs -> thb_sbp_new = A^{-1} rhs -> g, thb.
+// One substep's 1-D solve: flux -> rhs -> thb_sbp_new = A^{-1} rhs (banded
+// forward and back substitution with the preamble's LU, one thread) -> g, thb.
 __global__ void __launch_bounds__(1024)
 sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
                    const double* __restrict__ thb2s, const double* __restrict__ s2thb,
-                   double mu_sdt, const double* __restrict__ Bg, const double* __restrict__ ainv,
+                   double mu_sdt, const double* __restrict__ Bg, const double* __restrict__ LUg,
                    double* __restrict__ thb_io, double* __restrict__ g_out) {
   __shared__ double fl[SBP_NMAX], tb[SBP_NMAX], rh[SBP_NMAX], nw[SBP_NMAX], th0[SBP_NMAX];
+  __shared__ double LU[6 * SBP_NMAX], Bs[5 * SBP_NMAX], d1s[3 * SBP_NMAX];
   const int tid = threadIdx.x;
   for (int i = tid; i < n; i += blockDim.x) th0[i] = thb_io[i];
+  for (int i = tid; i < 6 * n; i += blockDim.x) LU[i] = LUg[i];
+  for (int i = tid; i < 5 * n; i += blockDim.x) Bs[i] = Bg[i];
+  for (int i = tid; i < 3 * n; i += blockDim.x) d1s[i] = d1b[i];
   for (int i = tid; i < n; i += blockDim.x) {
-    double s = 0.0;
-    for (int p = 0; p < P; ++p) s += partial[(size_t)i * P + p];
-    fl[i] = s;
+    double sum = 0.0;
+    for (int p = 0; p < P; ++p) sum += partial[(size_t)i * P + p];
+    fl[i] = sum;
   }
   __syncthreads();
   block_matvec(n, n, thb2s, th0, tb);
@@ -500,24 +500,73 @@ sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const doubl
     double r = 0.0;
     for (int o = -2; o <= 2; ++o) {
       const int j = i + o;
-      if (j >= 0 && j < n) r += Bg[i * 5 + o + 2] * tb[j];
+      if (j >= 0 && j < n) r += Bs[i * 5 + o + 2] * tb[j];
     }
     double d = 0.0;
     for (int o = -1; o <= 1; ++o) {
       const int j = i + o;
-      if (j >= 0 && j < n) d += band3(d1b, i, j, n) * fl[j];
+      if (j >= 0 && j < n) d += band3(d1s, i, j, n) * fl[j];
     }
     r -= mu_sdt * d;
     rh[i] = (i == 0 || i == n - 1) ? 0.0 : r;
   }
   __syncthreads();
-  block_matvec(n, n, ainv, rh, nw);
+  if (tid == 0) {
+    // L y = rh (unit lower, multipliers at the a, b slots), U x = y; in
+    // chunks of 8 rows whose LDS operands are loaded before the dependent
+    // chain runs through them
+    constexpr int C = 8;
+    const double* rdp = LU + 5 * n;
+    double y1 = 0.0, y2 = 0.0;
+    for (int i0 = 0; i0 < n; i0 += C) {
+      double r[C], l1[C], l0[C];
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const int i = min(i0 + u, n - 1);
+        r[u] = rh[i];
+        l1[u] = LU[i * 5 + 1];
+        l0[u] = LU[i * 5 + 0];
+      }
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        if (i0 + u < n) {
+          const double v = r[u] - (l1[u] * y1 + l0[u] * y2);
+          nw[i0 + u] = v;
+          y2 = y1;
+          y1 = v;
+        }
+      }
+    }
+    // x_i = (y_i - e_i x_{i+1} - f_i x_{i+2}) / d_i; e, f of the last rows
+    // are zero (band entries outside the matrix)
+    double x1 = 0.0, x2 = 0.0;
+    for (int i0 = n - 1; i0 >= 0; i0 -= C) {
+      double yv[C], ev[C], fv[C], rv[C];
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const int i = max(i0 - u, 0);
+        yv[u] = nw[i];
+        ev[u] = LU[i * 5 + 3];
+        fv[u] = LU[i * 5 + 4];
+        rv[u] = rdp[i];
+      }
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        if (i0 - u >= 0) {
+          const double v = (yv[u] - ev[u] * x1 - fv[u] * x2) * rv[u];
+          nw[i0 - u] = v;
+          x2 = x1;
+          x1 = v;
+        }
+      }
+    }
+  }
   __syncthreads();
   for (int i = tid; i < n; i += blockDim.x) {
     double gs = 0.0;
     for (int o = -1; o <= 1; ++o) {
       const int j = i + o;
-      if (j >= 0 && j < n) gs += band3(d1b, i, j, n) * (tb[j] + nw[j]);
+      if (j >= 0 && j < n) gs += band3(d1s, i, j, n) * (tb[j] + nw[j]);
     }
     g_out[i] = 0.5 * gs;
   }
@@ -530,16 +579,16 @@ sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const doubl
   block_matvec(n, n, s2thb, nw, thb_io);
 }
 
-// sbp buffer (doubles, n = Nz+1): [0,5n) B band, [16n, 16n + n*n) A^{-1}
-// row-major.
+// sbp buffer (doubles, n = Nz+1): [0,5n) B band, [5n,10n) LU bands,
+// [10n,11n) reciprocal pivots.
 hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double* d1b,
                               const double* lb, double c1, double c2, double* sbp,
                               hipStream_t s) {
   if (n > SBP_NMAX) return hipErrorInvalidValue;
   double* Bg = sbp;
-  double* ainv = sbp + 16 * (size_t)n;
+  double* LUg = sbp + 5 * (size_t)n;
   hipLaunchKernelGGL(sbp_prepare_kernel, dim3(1), dim3(512), 0, s, n, P, partial, d1b, lb, c1, c2,
-                     Bg, ainv);
+                     Bg, LUg);
   return hipGetLastError();
 }
 
@@ -548,9 +597,9 @@ hipError_t launch_sbp_substep(int n, int P, const double* partial, const double*
                               double* sbp, double* thb_io, double* g_out, hipStream_t s) {
   if (n > SBP_NMAX) return hipErrorInvalidValue;
   const double* Bg = sbp;
-  const double* ainv = sbp + 16 * (size_t)n;
+  const double* LUg = sbp + 5 * (size_t)n;
   hipLaunchKernelGGL(sbp_substep_kernel, dim3(1), dim3(1024), 0, s, n, P, partial, d1b, thb2s,
-                     s2thb, mu_sdt, Bg, ainv, thb_io, g_out);
+                     s2thb, mu_sdt, Bg, LUg, thb_io, g_out);
   return hipGetLastError();
 }
 

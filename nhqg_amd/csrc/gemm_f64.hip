@@ -33,6 +33,60 @@
 
 static_assert(GK_KC % 4 == 0, "k-chunk must be a multiple of the MFMA depth");
 
+// Epilogue of tiles [t0, t0 + 4) of one lane (16 columns per wave, rows
+// t*16 + r*4 + lk): the same per-element arithmetic as epi_apply, but each
+// extra field is read for all 16 elements at once, so a term costs one
+// memory latency rather than one per element.  Rows beyond M are clamped
+// for the loads and not stored.
+template <int MT>
+NHQG_DEV void epi_store4(const Epi& e, double* C, const v4d (&acc)[MT], int t0, int M, int ld,
+                         int col, int lk, int mode) {
+  double o[4][4];
+  const double sa = e.s_acc ? e.s_acc[mode] : 1.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      o[t][r] = e.a_acc * (t0 + t < MT ? acc[t0 + t < MT ? t0 + t : 0][r] : 0.0);
+      if (e.s_acc) o[t][r] *= sa;
+    }
+  for (int x = 0; x < e.nX; ++x) {
+    const double* X = e.X[x];
+    const double ax = e.aX[x];
+    const bool hs = e.sX[x] != nullptr;
+    const double sx = hs ? e.sX[x][mode] : 1.0;
+    double v[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = min((t0 + t) * 16 + r * 4 + lk, M - 1);
+        v[t][r] = X[(size_t)row * ld + col];
+      }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double w = ax * v[t][r];
+        if (hs) w *= sx;
+        o[t][r] += w;
+      }
+  }
+  const double so = e.s_out ? e.s_out[mode] : 1.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = (t0 + t) * 16 + r * 4 + lk;
+      if (t0 + t < MT && row < M) {
+        double w = o[t][r];
+        if (e.s_out) w *= so;
+        if (e.d_tab) w *= e.d_tab[(size_t)row * e.d_ld + mode];
+        C[(size_t)row * ld + col] = w;
+      }
+    }
+}
+
 template <int MT, bool UPD>
 __global__ void __launch_bounds__(GK_NW * 64)
 gemm_f64_kernel(GemmBatch bt) {
@@ -135,18 +189,12 @@ gemm_f64_kernel(GemmBatch bt) {
   // v_mfma_f64_16x16x4_f64 output: D[i][j] of a tile sits in lane
   // j + 16*(i%4), register i/4 (one row group per register).
   const int mode = col >> 1;
+  if (J.C2) {
 #pragma unroll
-  for (int t = 0; t < MT; ++t) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = t * 16 + r * 4 + lk;
-      if (row < M) {
-        const size_t idx = (size_t)row * ld + col;
-        J.C[idx] = epi_apply(J.e, acc[t][r], idx, row, mode);
-        if (J.C2) J.C2[idx] = epi_apply(J.e2, acc[t][r], idx, row, mode);
-      }
-    }
+    for (int t0 = 0; t0 < MT; t0 += 4) epi_store4<MT>(J.e2, J.C2, acc, t0, M, ld, col, lk, mode);
   }
+#pragma unroll
+  for (int t0 = 0; t0 < MT; t0 += 4) epi_store4<MT>(J.e, J.C, acc, t0, M, ld, col, lk, mode);
   if (M > MP) {
     // the four k-residue partial sums sit in lanes li, li+16, li+32, li+48
     last += __shfl_xor(last, 16);
