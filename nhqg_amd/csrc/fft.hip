@@ -226,6 +226,53 @@ NHQG_DEV void inv_pair(double2* ba, double2* bb, int t, const double2* tw, int n
   __syncthreads();
 }
 
+// Half spectra P, Q (entries ky < nky) at the packed slots k = t + j*N/4 a
+// thread fills, loaded into registers ahead of use.
+template <int N>
+struct Half4 {
+  double2 p[4], q[4];
+};
+
+template <int N>
+NHQG_DEV void load_half(Half4<N>& h, int t, int nky, const double2* P, const double2* Q) {
+  constexpr int T = N / 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = t + j * T;
+    const int ky = (k <= N / 2) ? k : N - k;
+    const bool in = ky < nky;
+    h.p[j] = in ? P[ky] : make_double2(0.0, 0.0);
+    h.q[j] = (in && Q) ? Q[ky] : make_double2(0.0, 0.0);
+  }
+}
+
+// inverse of the packed spectrum built from registers (irfft convention on
+// ky = 0 and ky = N/2): p(y) + i q(y) at y = t + j*N/4
+template <int N>
+NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, const Half4<N>& h,
+                            double* p, double* q, double scale) {
+  constexpr int T = N / 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = t + j * T;
+    const double2 a = h.p[j], b = h.q[j];
+    double2 v;
+    if (k == 0 || k == N / 2) v = make_double2(a.x, b.x);
+    else if (k < N / 2) v = make_double2(a.x - b.y, a.y + b.x);
+    else v = make_double2(a.x + b.y, b.x - a.y);
+    ba[k] = v;
+  }
+  __syncthreads();
+  const double2* r = team_fft<N, true>(ba, bb, t, tw);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double2 z = r[t + j * T];
+    p[j] = z.x * scale;
+    if (q) q[j] = z.y * scale;
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Pass 2: per (level, x) row.  Y6[g][l][x][ky], g = Gq, Hq, Gw, Hw, Gth, Hth
 // with G_f = F_y(u f) and H_f = i ky F_y(v f) + F_y(sigma f).
@@ -253,23 +300,34 @@ row_flux(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double
   // unpacking leaks roundoff of one member into the other, which is
   // harmless at equal scale (u with v; u f with v f) but not across fields
   // of very different size (a strong barotropic q next to a weak w).
-  double u[4], v[4], q[4], w[4], th[4];
-  inv_pair<N>(ba, bb, t, tw, nky, X5 + 0 * fstride + ro, X5 + 1 * fstride + ro, u, v, sc);
-  inv_pair<N>(ba, bb, t, tw, nky, X5 + 2 * fstride + ro, nullptr, q, nullptr, sc);
-  inv_pair<N>(ba, bb, t, tw, nky, X5 + 3 * fstride + ro, nullptr, w, nullptr, sc);
-  inv_pair<N>(ba, bb, t, tw, nky, X5 + 4 * fstride + ro, nullptr, th, nullptr, sc);
-  if (TRAP) {
+  // Each X5 row is loaded into registers one transform ahead of its use.
+  double eta[4], sig[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) q[j] += eta_phys[(size_t)x * N + t + j * T];
+  for (int j = 0; j < 4; ++j) {
+    eta[j] = TRAP ? eta_phys[(size_t)x * N + t + j * T] : 0.0;
+    sig[j] = SPONGE ? sponge_phys[(size_t)x * N + t + j * T] : 0.0;
   }
-  double* fld[3] = {q, w, th};
-#pragma unroll
+  Half4<N> hc, hn;
+  load_half<N>(hc, t, nky, X5 + 0 * fstride + ro, X5 + 1 * fstride + ro);
+  load_half<N>(hn, t, nky, X5 + 2 * fstride + ro, nullptr);
+  double u[4], v[4];
+  inv_pair_regs<N>(ba, bb, t, tw, hc, u, v, sc);
+#pragma unroll 1
   for (int f = 0; f < 3; ++f) {
+    hc = hn;
+    if (f < 2) load_half<N>(hn, t, nky, X5 + (3 + f) * fstride + ro, nullptr);
+    else if (SPONGE) load_half<N>(hn, t, nky, X5 + 5 * fstride + ro, nullptr);
+    double fl[4];
+    inv_pair_regs<N>(ba, bb, t, tw, hc, fl, nullptr, sc);
+    if (TRAP && f == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) fl[j] += eta[j];
+    }
     double a[4], b[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      a[j] = u[j] * fld[f][j];
-      b[j] = v[j] * fld[f][j];
+      a[j] = u[j] * fl[j];
+      b[j] = v[j] * fl[j];
     }
     double2 G[3], Hv[3];
     fwd_pair<N>(ba, bb, t, tw, a, b, nky, G, Hv);
@@ -288,13 +346,15 @@ row_flux(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double
   }
   if (SPONGE) {
     // sigma times each coefficient level of q', w, theta (spectral.py:53-60)
-#pragma unroll
+#pragma unroll 1
     for (int f = 0; f < 3; ++f) {
-      if (f > 0 && l >= ng) continue;  // block-uniform
+      if (f > 0 && l >= ng) break;  // block-uniform
+      hc = hn;
+      if (f < 2 && l < ng) load_half<N>(hn, t, nky, X5 + (6 + f) * fstride + ro, nullptr);
       double c[4], a[4];
-      inv_pair<N>(ba, bb, t, tw, nky, X5 + (5 + f) * fstride + ro, nullptr, c, nullptr, sc);
+      inv_pair_regs<N>(ba, bb, t, tw, hc, c, nullptr, sc);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) a[j] = sponge_phys[(size_t)x * N + t + j * T] * c[j];
+      for (int j = 0; j < 4; ++j) a[j] = sig[j] * c[j];
       double2 S[3], dummy[3];
       fwd_pair<N>(ba, bb, t, tw, a, nullptr, nky, S, dummy);
       if (active) {
@@ -333,38 +393,67 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
   double2* outs[3] = {out_q, out_w, out_th};
   double2* sps[3] = {sp_q, sp_w, sp_th};
   constexpr int NG = SPONGE ? 3 : 2;  // G, H and (sponge) S per field
+  constexpr int NQ = 3 * NG;
+  constexpr int NL = N * TEAMS / 256; // slot values gathered per thread
+  // output rows, kx and mask of this thread's modes, the same for every field
+  int rowv[4];
+  double kxv[4], mkv[4];
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    double2 g[4];
+  for (int j = 0; j < 4; ++j) {
+    const int i = t + j * T;
+    const bool in = i < nkx;
+    rowv[j] = in ? rows[i] : 0;
+    kxv[j] = in ? kx_row[i] : 0.0;
+    mkv[j] = (in && active) ? mask[ky * nkx + i] : 0.0;
+  }
+  // the sponge slot of w and theta exists only on their ng levels
+  auto valid = [&](int q) { return !(q % NG == 2 && q / NG > 0 && l >= ng); };
+  auto gather = [&](int q, double2 (&v)[NL]) {
+    const int f = q / NG, h = q % NG;
+    const int slot = (h < 2) ? 2 * f + h : 6 + f;
+    const double2* src = Y6 + slot * fstride + (size_t)l * N * nky;
 #pragma unroll
-    for (int h = 0; h < NG; ++h) {
-      if (h == 2 && f > 0 && l >= ng) continue;  // block-uniform
-      const int slot = (h < 2) ? 2 * f + h : 6 + f;
-      const double2* src = Y6 + slot * fstride + (size_t)l * N * nky;
-      for (int idx = tid; idx < N * TEAMS; idx += 256) {
-        const int cc = idx % TEAMS, x = idx / TEAMS;
-        const int kyc = ky0 + cc;
-        buf[0][cc][x] = (kyc < nky) ? src[(size_t)x * nky + kyc] : make_double2(0.0, 0.0);
-      }
-      __syncthreads();
-      const double2* r = team_fft<N, false>(buf[0][team], buf[1][team], t, tw);
+    for (int it = 0; it < NL; ++it) {
+      const int idx = tid + it * 256;
+      const int cc = idx % TEAMS, x = idx / TEAMS;
+      const int kyc = ky0 + cc;
+      v[it] = (kyc < nky) ? src[(size_t)x * nky + kyc] : make_double2(0.0, 0.0);
+    }
+  };
+  double2 cur[NL], nxt[NL], g[4];
+  gather(0, nxt);
+#pragma unroll 1
+  for (int q = 0; q < NQ; ++q) {
+    if (!valid(q)) continue;  // block-uniform
+    const int f = q / NG, h = q % NG;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int i = t + j * T;
-        if (i < nkx) {
-          const double2 z = r[rows[i]];
-          const int m = ky * nkx + i;
-          if (h == 0) {
-            g[j] = cmul_i(z, kx_row[i]);
-          } else if (h == 1) {
-            if (active) outs[f][(size_t)l * ldc + m] = cscale(cadd(g[j], z), mask[m]);
-          } else if (active) {
-            sps[f][(size_t)l * ldc + m] = cscale(z, mask[m]);
-          }
+    for (int it = 0; it < NL; ++it) cur[it] = nxt[it];
+    int qn = q + 1;
+    while (qn < NQ && !valid(qn)) ++qn;
+    if (qn < NQ) gather(qn, nxt);
+#pragma unroll
+    for (int it = 0; it < NL; ++it) {
+      const int idx = tid + it * 256;
+      buf[0][idx % TEAMS][idx / TEAMS] = cur[it];
+    }
+    __syncthreads();
+    const double2* r = team_fft<N, false>(buf[0][team], buf[1][team], t, tw);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = t + j * T;
+      if (i < nkx) {
+        const double2 z = r[rowv[j]];
+        const int m = ky * nkx + i;
+        if (h == 0) {
+          g[j] = cmul_i(z, kxv[j]);
+        } else if (h == 1) {
+          if (active) outs[f][(size_t)l * ldc + m] = cscale(cadd(g[j], z), mkv[j]);
+        } else if (active) {
+          sps[f][(size_t)l * ldc + m] = cscale(z, mkv[j]);
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
 }
 
