@@ -198,6 +198,44 @@ int nhqg_stream_copy(nhqg_ctx *ctx, uint64_t bytes, int32_t reps, double *gbps);
  * (imex_inv[s] of nhqg/grid.py:430, Ng x Ng row-major) to out. */
 int nhqg_get_shell_inverse(nhqg_ctx *ctx, int32_t s, double *out);
 
+/* ---- Learned vertical-tail closure (no counterpart in the reference) ----
+ * A per-column MLP x (16) -> tanh (128) -> tanh (128) -> 16 acting on the
+ * top 16 Chebyshev coefficients of q; its fallback and judge is the plain
+ * solver at higher Nz.  Parameters: one flat float32 array W1 (128 x 16),
+ * b1, W2 (128 x 128), b2, W3 (16 x 128), b3.  Training on the device:
+ * FP32 forward/backward, mean-squared loss, Adam.
+ * NHQG_MLP_ODD holds every bias at zero (set_params clears them, training
+ * leaves them): the network is then odd, MLP(-x) = -MLP(x), so the step
+ * closure adds nothing to zero columns and keeps the k / -k columns of
+ * ky = 0 Hermitian. */
+#define NHQG_MLP_ODD 1
+typedef struct nhqg_mlp nhqg_mlp;
+int nhqg_mlp_create(int32_t n_in, int32_t width, int32_t n_out, int32_t max_batch,
+                    int32_t flags, nhqg_mlp **out);
+int nhqg_mlp_destroy(nhqg_mlp *m);
+int nhqg_mlp_n_params(const nhqg_mlp *m, int64_t *n);
+int nhqg_mlp_set_params(nhqg_mlp *m, const float *w);  /* also resets Adam */
+int nhqg_mlp_get_params(nhqg_mlp *m, float *w);
+/* y (batch x n_out) = MLP(x (batch x n_in)) */
+int nhqg_mlp_forward(nhqg_mlp *m, const float *x, int32_t batch, float *y);
+/* gradient of mean((MLP(x) - t)^2) w.r.t. the parameters, and the loss */
+int nhqg_mlp_gradient(nhqg_mlp *m, const float *x, const float *t, int32_t batch, float *grad,
+                      float *loss);
+/* training set resident on the device (n samples) */
+int nhqg_mlp_set_data(nhqg_mlp *m, const float *x, const float *t, int32_t n);
+/* n_steps Adam steps over consecutive minibatches starting at sample start;
+ * loss[i] (may be NULL) = minibatch loss before step i */
+int nhqg_mlp_train(nhqg_mlp *m, int32_t n_steps, int32_t batch, int32_t start, float lr,
+                   float *loss);
+const char *nhqg_mlp_last_error(const nhqg_mlp *m);
+/* Attach the network (BF16-rounded weights, FP32 accumulation) to every
+ * explicit RHS of the step: E_q[n0 + j] += out_scale * MLP(in_scale *
+ * q[n0 .. n0+15])_j for each mode and each of Re, Im.  The network must be
+ * 16 -> 128 -> 128 -> 16 and created with NHQG_MLP_ODD (NHQG_ERR_ARG
+ * otherwise); mlp = NULL detaches. */
+int nhqg_set_closure(nhqg_ctx *ctx, const nhqg_mlp *mlp, int32_t n0, double in_scale,
+                     double out_scale);
+
 /* Bytes of device memory held by the context. */
 int nhqg_device_bytes(nhqg_ctx *ctx, uint64_t *bytes);
 

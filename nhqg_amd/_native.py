@@ -22,6 +22,7 @@ OP_IMPLICIT_TEND = 2
 OP_IMPLICIT_SOLVE = 3
 OP_CORRECTOR = 4
 OP_SANITIZE = 5
+MLP_ODD = 1
 DIAG_NAMES = ("KE_tot", "KE_bt", "KE_bc", "max_speed", "enstrophy", "Nusselt",
               "Nusselt_dealiased", "vol_avg_tw_dealiased", "th_bar_max", "q_rms", "w_rms",
               "th_rms")
@@ -64,6 +65,9 @@ EXPORTS = (
     "nhqg_last_step_ms", "nhqg_profile_step", "nhqg_timing_name", "nhqg_device_bytes",
     "nhqg_get_shell_inverse",
     "nhqg_last_error", "nhqg_stream_copy",
+    "nhqg_mlp_create", "nhqg_mlp_destroy", "nhqg_mlp_n_params", "nhqg_mlp_set_params",
+    "nhqg_mlp_get_params", "nhqg_mlp_forward", "nhqg_mlp_gradient", "nhqg_mlp_set_data",
+    "nhqg_mlp_train", "nhqg_mlp_last_error", "nhqg_set_closure",
 )
 
 
@@ -101,6 +105,20 @@ def load():
     lib.nhqg_timing_name.restype = C.c_char_p
     lib.nhqg_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
     lib.nhqg_get_shell_inverse.argtypes = [vp, C.c_int32, _dp]
+    _fp = C.POINTER(C.c_float)
+    lib.nhqg_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    C.POINTER(vp)]
+    lib.nhqg_mlp_destroy.argtypes = [vp]
+    lib.nhqg_mlp_n_params.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.nhqg_mlp_set_params.argtypes = [vp, _fp]
+    lib.nhqg_mlp_get_params.argtypes = [vp, _fp]
+    lib.nhqg_mlp_forward.argtypes = [vp, _fp, C.c_int32, _fp]
+    lib.nhqg_mlp_gradient.argtypes = [vp, _fp, _fp, C.c_int32, _fp, _fp]
+    lib.nhqg_mlp_set_data.argtypes = [vp, _fp, _fp, C.c_int32]
+    lib.nhqg_mlp_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, _fp]
+    lib.nhqg_mlp_last_error.argtypes = [vp]
+    lib.nhqg_mlp_last_error.restype = C.c_char_p
+    lib.nhqg_set_closure.argtypes = [vp, vp, C.c_int32, C.c_double, C.c_double]
     lib.nhqg_stream_copy.argtypes = [vp, C.c_uint64, C.c_int32, _dp]
     lib.nhqg_last_error.argtypes = [vp]
     lib.nhqg_last_error.restype = C.c_char_p
@@ -270,7 +288,96 @@ class Context:
                     "nhqg_get_shell_inverse")
         return out
 
+    def set_closure(self, mlp, n0: int, in_scale: float, out_scale: float) -> None:
+        """Attach a trained MLP (BF16 weights) to every explicit RHS of the
+        step; mlp=None detaches."""
+        self._check(self.lib.nhqg_set_closure(self.h, mlp.h if mlp is not None else None,
+                                              int(n0), float(in_scale), float(out_scale)),
+                    "nhqg_set_closure")
+
     def device_bytes(self):
         v = C.c_uint64()
         self._check(self.lib.nhqg_device_bytes(self.h, C.byref(v)), "nhqg_device_bytes")
         return int(v.value)
+
+
+class MLP:
+    """Device MLP of the learned closure (nhqg_mlp_*): FP32 parameters,
+    forward / gradient / Adam training on the GPU."""
+
+    def __init__(self, n_in: int = 16, width: int = 128, n_out: int = 16, max_batch: int = 4096,
+                 odd: bool = False):
+        self.lib = load()
+        self.n_in, self.width, self.n_out, self.max_batch = n_in, width, n_out, max_batch
+        self.odd = bool(odd)
+        h = C.c_void_p()
+        rc = self.lib.nhqg_mlp_create(n_in, width, n_out, max_batch, MLP_ODD if odd else 0,
+                                      C.byref(h))
+        if rc != NHQG_OK:
+            raise NativeError(f"nhqg_mlp_create failed ({rc})")
+        self.h = h
+        n = C.c_int64()
+        self._check(self.lib.nhqg_mlp_n_params(self.h, C.byref(n)), "nhqg_mlp_n_params")
+        self.n_params = int(n.value)
+
+    def _check(self, rc, what):
+        if rc != NHQG_OK:
+            raise NativeError(f"{what} failed ({rc}): {self.lib.nhqg_mlp_last_error(self.h).decode()}")
+
+    @staticmethod
+    def _f(a):
+        return np.ascontiguousarray(a, dtype=np.float32)
+
+    def set_params(self, w):
+        w = self._f(w)
+        assert w.size == self.n_params
+        self._check(self.lib.nhqg_mlp_set_params(self.h, _ptr(w, C.POINTER(C.c_float))),
+                    "nhqg_mlp_set_params")
+
+    def get_params(self):
+        w = np.empty(self.n_params, np.float32)
+        self._check(self.lib.nhqg_mlp_get_params(self.h, _ptr(w, C.POINTER(C.c_float))),
+                    "nhqg_mlp_get_params")
+        return w
+
+    def forward(self, x):
+        x = self._f(x)
+        y = np.empty((x.shape[0], self.n_out), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.nhqg_mlp_forward(self.h, _ptr(x, fp), x.shape[0], _ptr(y, fp)),
+                    "nhqg_mlp_forward")
+        return y
+
+    def gradient(self, x, t):
+        x, t = self._f(x), self._f(t)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.nhqg_mlp_gradient(self.h, _ptr(x, fp), _ptr(t, fp), x.shape[0],
+                                               _ptr(g, fp), C.byref(loss)), "nhqg_mlp_gradient")
+        return g, float(loss.value)
+
+    def set_data(self, x, t):
+        x, t = self._f(x), self._f(t)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.nhqg_mlp_set_data(self.h, _ptr(x, fp), _ptr(t, fp), x.shape[0]),
+                    "nhqg_mlp_set_data")
+        self.n_data = x.shape[0]
+
+    def train(self, n_steps: int, batch: int, lr: float, start: int = 0):
+        loss = np.empty(n_steps, np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.nhqg_mlp_train(self.h, n_steps, batch, start, lr, _ptr(loss, fp)),
+                    "nhqg_mlp_train")
+        return loss
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nhqg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -75,6 +75,11 @@ struct nhqg_ctx {
   };
   bool prof = false;
   std::vector<PRec> recs;
+  // learned closure (nhqg_set_closure): BF16 weights, top-level offset, scales
+  void* cl_w = nullptr;
+  bool cl_on = false;
+  int cl_n0 = 0;
+  float cl_in = 1.0f, cl_out = 1.0f;
 };
 
 namespace {
@@ -300,6 +305,9 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   if (c->P.beta != 0.0)
     LK(launch_beta(c->Nq, c->ldc, c->nm, c->P.beta, o.kx_mode, o.inv_denom, (const double2*)q,
                    (double2*)Eq, c->st));
+  if (c->cl_on)
+    PL(CL_SMALL, 0.0,
+       launch_closure(2 * c->nm, c->ld, c->cl_n0, c->cl_w, q, Eq, c->cl_in, c->cl_out, c->st));
   return NHQG_OK;
 }
 
@@ -946,6 +954,7 @@ int nhqg_destroy(nhqg_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->cl_w) (void)hipFree(c->cl_w);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -1158,6 +1167,31 @@ int nhqg_get_shell_inverse(nhqg_ctx* c, int32_t s, double* out) {
   // the slot holds inv_s[j][i] at j*np + i
   for (int i = 0; i < Ng; ++i)
     for (int j = 0; j < Ng; ++j) out[(size_t)i * Ng + j] = t[(size_t)j * np + i];
+  return NHQG_OK;
+}
+
+int nhqg_set_closure(nhqg_ctx* c, const nhqg_mlp* mlp, int32_t n0, double in_scale,
+                     double out_scale) {
+  if (!c) return NHQG_ERR_ARG;
+  if (!mlp) {
+    c->cl_on = false;
+  } else {
+    if (n0 < 0 || n0 + 16 > c->Nq) {
+      c->err = "closure levels n0 .. n0+15 must lie within the Nz+1 q levels";
+      return NHQG_ERR_ARG;
+    }
+    int r = closure_export_bf16(mlp, &c->cl_w, c->st, c->err);
+    if (r) return r;
+    c->cl_on = true;
+    c->cl_n0 = n0;
+    c->cl_in = (float)in_scale;
+    c->cl_out = (float)out_scale;
+  }
+  // the captured step graph no longer matches
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
   return NHQG_OK;
 }
 

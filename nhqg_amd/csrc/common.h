@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <string>
 
 #define NHQG_DEV __device__ __forceinline__
 
@@ -173,5 +174,11 @@ void gemm_pack_op(const double* A, int M, int K, int mt, double* out);
 // Runs b.job[0..nj) in one launch; every job has the same mt and all or
 // none use the update.  ld must be a multiple of 128.
 hipError_t launch_gemm_batch(const GemmBatch& b, int nj, hipStream_t s);
+// Learned closure (closure.hip): E_q[n0 + j][c] += out_scale * MLP(in_scale *
+// q[n0 .. n0+15][c]) for every column c < nm2 of the level-major fields.
+hipError_t launch_closure(int nm2, int ld, int n0, const void* wb, const double* q, double* Eq,
+                          float in_scale, float out_scale, hipStream_t s);
+struct nhqg_mlp;
+int closure_export_bf16(const nhqg_mlp* m, void** dst, hipStream_t s, std::string& err);
 // out[l][c] = epilogue(0) over nl rows of ld doubles.
 hipError_t launch_combine(int nl, int ld, double* out, const Epi& e, hipStream_t s);
