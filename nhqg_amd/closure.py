@@ -162,3 +162,164 @@ def psi_ke_spectrum(q_hat: np.ndarray, g: Grid) -> np.ndarray:
         wr[-1] = 1.0
     psi = -q_hat * g.inv_denom[None]
     return np.sum(g.ksq[None] * np.abs(psi) ** 2 * wr[None, None], axis=(1, 2)) / float(g.Nx) ** 4
+
+
+def _filtered_diagnostics(gd: Grid, gf: Grid, sf: State) -> dict:
+    """Diagnostics of the fine state restricted to the coarse degrees
+    n <= Nz_coarse (evaluated on the coarse grid's device context)."""
+    from nhqg_amd.diagnostics import unpack
+
+    ctx = gd.context(full=False)
+    ctx.set_state(*filter_state(sf, gf, gd))
+    return unpack(ctx.diagnostics(), gd.Nz + 1)
+
+
+def _spectrum_errors(e_c: np.ndarray, e_f: np.ndarray) -> dict:
+    top = slice(e_c.size - N_TOP, e_c.size)
+    return {"psi_ke_spectrum": float(np.abs(e_c - e_f).sum() / e_f.sum()),
+            "psi_ke_top16": float(np.abs(e_c[top] - e_f[top]).sum() / e_f[top].sum())}
+
+
+def judge(cfg_f, cfg_c, state0: State, n_steps: int, every: int, closure: Closure | None = None
+          ) -> dict:
+    """Coarse run (plain, and with the closure when given) against the plain
+    fine run from the same state.
+
+    Every `every` steps: psi-KE per Chebyshev degree n <= Nz_coarse
+    (relative L1 error over all coarse degrees and over the top 16),
+    max_speed of the fine state restricted to n <= Nz_coarse against the
+    coarse run's, and Nu_d of each run.  Returns the series and the means
+    over the checkpoints."""
+    from nhqg_amd.grid import make_grid
+
+    gf = make_grid(cfg_f, shells="band")
+    runs = {"coarse": make_grid(cfg_c, shells="band")}
+    if closure is not None:
+        runs["coarse+closure"] = make_grid(cfg_c, shells="band")
+    gd = make_grid(cfg_c, shells="band")
+    fine = Stepper(gf)
+    fine.set_state(state0)
+    s0 = fine.state()
+    steppers = {}
+    for name, g in runs.items():
+        sp = Stepper(g)
+        sp.set_state(filter_state(s0, gf, g))
+        if name == "coarse+closure":
+            closure.attach(g)
+        steppers[name] = sp
+    nqc = cfg_c.Nz + 1
+    series = {"step": []}
+    done = 0
+    while done < n_steps:
+        n = min(every, n_steps - done)
+        fine.run(n)
+        for sp in steppers.values():
+            sp.run(n)
+        done += n
+        sf = fine.state()
+        e_f = psi_ke_spectrum(sf.q_hat, gf)[:nqc]
+        fd = fine.diagnostics()
+        rd = _filtered_diagnostics(gd, gf, sf)
+        series["step"].append(done)
+        series.setdefault("fine.Nu_d", []).append(fd["Nusselt_dealiased"])
+        series.setdefault("fine.max_speed_resolved", []).append(rd["max_speed"])
+        series.setdefault("fine.KE_tot", []).append(fd["KE_tot"])
+        for name, sp in steppers.items():
+            cs = sp.state()
+            cd = sp.diagnostics()
+            errs = _spectrum_errors(psi_ke_spectrum(cs.q_hat, runs[name]), e_f)
+            series.setdefault(f"{name}.Nu_d", []).append(cd["Nusselt_dealiased"])
+            series.setdefault(f"{name}.max_speed", []).append(cd["max_speed"])
+            series.setdefault(f"{name}.KE_tot", []).append(cd["KE_tot"])
+            for k, v in errs.items():
+                series.setdefault(f"{name}.{k}_err", []).append(v)
+    out = {"series": series, "summary": {}}
+    nu_f = np.array(series["fine.Nu_d"])
+    ms_f = np.array(series["fine.max_speed_resolved"])
+    for name in steppers:
+        nu = np.array(series[f"{name}.Nu_d"])
+        ms = np.array(series[f"{name}.max_speed"])
+        out["summary"][name] = {
+            "psi_ke_spectrum_err_final": series[f"{name}.psi_ke_spectrum_err"][-1],
+            "psi_ke_spectrum_err_mean": float(np.mean(series[f"{name}.psi_ke_spectrum_err"])),
+            "psi_ke_top16_err_final": series[f"{name}.psi_ke_top16_err"][-1],
+            "psi_ke_top16_err_mean": float(np.mean(series[f"{name}.psi_ke_top16_err"])),
+            "Nu_d_rel_err_final": float(abs(nu[-1] - nu_f[-1]) / abs(nu_f[-1])),
+            "Nu_d_rel_err_mean": float(np.mean(np.abs(nu - nu_f) / np.abs(nu_f))),
+            "Nu_d_minus_1_rel_err_final": float(abs(nu[-1] - nu_f[-1]) / abs(nu_f[-1] - 1.0)),
+            "max_speed_rel_err_final": float(abs(ms[-1] - ms_f[-1]) / ms_f[-1]),
+            "max_speed_rel_err_mean": float(np.mean(np.abs(ms - ms_f) / ms_f)),
+        }
+    for g in (gf, gd, *runs.values()):
+        g.close()
+    return out
+
+
+def main(argv=None) -> int:
+    """Train on pairs from one seeded fine run, judge on a run from another
+    seed; prints one JSON object."""
+    import argparse
+    import json
+    import time
+
+    from nhqg_amd.grid import make_grid
+    from nhqg_amd.init import add_barotropic_flow, polar_config_256, thermal_noise_state
+
+    ap = argparse.ArgumentParser(prog="python -m nhqg_amd.closure")
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--nz-fine", type=int, default=256)
+    ap.add_argument("--nz-coarse", type=int, default=64)
+    ap.add_argument("--theta-amp", type=float, default=1e-2)
+    ap.add_argument("--u-rms", type=float, default=15.0)
+    ap.add_argument("--pairs", type=int, default=10000)
+    ap.add_argument("--spinup", type=int, default=500)
+    ap.add_argument("--snapshots", type=int, default=10)
+    ap.add_argument("--snap-every", type=int, default=450)
+    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--every", type=int, default=250)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed-train", type=int, default=0)
+    ap.add_argument("--seed-judge", type=int, default=10)
+    args = ap.parse_args(argv)
+
+    cfg_f = polar_config_256(Nx=args.nx, Nz=args.nz_fine)
+    cfg_c = polar_config_256(Nx=args.nx, Nz=args.nz_coarse)
+
+    def seeded(g, seed):
+        s = thermal_noise_state(g, seed=seed, amplitude=args.theta_amp)
+        return add_barotropic_flow(s, g, seed=seed + 1, u_rms=args.u_rms)
+
+    t0 = time.perf_counter()
+    gf, gc = make_grid(cfg_f, shells="band"), make_grid(cfg_c, shells="band")
+    pairs = make_pairs(gf, gc, seeded(gf, args.seed_train), spinup=args.spinup,
+                       snapshots=args.snapshots, every=args.snap_every, seed=args.seed_train,
+                       n_samples=args.pairs)
+    t_pairs = time.perf_counter() - t0
+    print(f"pairs {pairs.x.shape[0]} in {t_pairs:.1f} s", flush=True)
+    gf.close()
+    gc.close()
+    t0 = time.perf_counter()
+    c = train(pairs, steps=args.train_steps, batch=512, lr=args.lr, seed=args.seed_train)
+    t_train = time.perf_counter() - t0
+    print(f"trained in {t_train:.1f} s, loss {c.loss[:20].mean():.3e} -> {c.loss[-100:].mean():.3e}",
+          flush=True)
+    gj = make_grid(cfg_f, shells="band")
+    s_judge = seeded(gj, args.seed_judge)
+    gj.close()
+    t0 = time.perf_counter()
+    res = judge(cfg_f, cfg_c, s_judge, args.steps, args.every, closure=c)
+    t_judge = time.perf_counter() - t0
+    res["config"] = vars(args)
+    res["timing_s"] = {"pairs": t_pairs, "train": t_train, "judge": t_judge}
+    res["train_loss"] = {"first20": float(c.loss[:20].mean()),
+                         "last100": float(c.loss[-100:].mean())}
+    print(json.dumps(res["summary"], indent=1), flush=True)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main())

@@ -134,6 +134,30 @@ def test_one_step(name):
         assert rel(got.th_bar, ref.thb) < TOL
 
 
+WIDE = {
+    "trap_sponge_512": lambda: cases.trap_sponge_cfg(Nx=512, Nz=10),
+    "fixture_1024": lambda: cases.fixture_cfg(Nx=1024, Nz=8),
+}
+
+
+@pytest.mark.parametrize("name", list(WIDE))
+def test_one_step_wide(name):
+    """The Nx = 512 and 1024 instantiations of the FFT passes, at small Nz
+    so the oracle step stays at seconds."""
+    cfg = WIDE[name]()
+    g = make_grid(cfg)
+    o = O.build_operators(cfg)
+    s = cases.active_state(cfg)
+    say("one step", name)
+    got = imex_step(State(s.q, s.w, s.th, s.thb), g)
+    ref = O.imex_step(s, o)
+    assert rel(got.q_hat, ref.q) < TOL
+    assert rel(got.w_hat, ref.w) < TOL
+    assert rel(got.th_hat, ref.th) < TOL
+    assert rel(got.th_bar, ref.thb) < TOL
+    g.close()
+
+
 def test_golden_fixture_20_steps():
     """The reference's own 20-step regression state
     (tests/data/m2b_dirichlet_evolvemean_ref.npz of the reference), from
