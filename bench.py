@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "ARS222 steps/s at 256²×256: FP64 plain vs mixed/closure at stated error"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+PEAK_INT8_TOPS = 5000.0   # dense int8 MFMA
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_polar_256x256x256.md")
@@ -59,7 +60,7 @@ def profiled_traffic(cls: str):
     return tot / n if n else None
 
 
-def rooflines(prof: dict, g, lay, full_size: bool) -> dict:
+def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
     """Achieved rate of each kernel class in one profiled step, against the
     MI355X roofline that bounds it.  Work is algorithmic, in the units a
     launch processes: 4*M*K flops per complex band mode for a contraction;
@@ -68,7 +69,8 @@ def rooflines(prof: dict, g, lay, full_size: bool) -> dict:
     nq, ng, nm = g.Nz + 1, g.Nz - 1, lay.nm
     out = {}
     ms, n, work = prof["gemm_f64_mfma"]
-    ach = work / (ms * 1e-3) / 1e12
+    n = max(n, 1)
+    ach = work / max(ms * 1e-3, 1e-30) / 1e12
     out["gemm_f64_mfma"] = dict(
         kernel="gemm_f64_kernel (batched vertical contractions)", launches_per_step=n,
         ms_per_step=ms, avg_launch_us=1e3 * ms / n, bound="mfma", achieved=ach,
@@ -89,6 +91,19 @@ def rooflines(prof: dict, g, lay, full_size: bool) -> dict:
         work="6 fields (q, w, theta nodal in; 3 tendencies out) x (Nz+1) levels x band modes x 16 B "
              "per RHS",
         traffic=profiled_traffic("flux") if full_size else None)
+    ms, n, work = prof.get("gemm_i8_mixed", (0.0, 0, 0.0))
+    if n > 0:
+        npair = slices * (slices + 1) // 2
+        ach = work / (ms * 1e-3) / 1e12
+        ops = npair * work / (ms * 1e-3) / 1e12
+        out["gemm_i8_mixed"] = dict(
+            kernel=f"gemm_i8_kernel<{slices}> (int8 slice contractions, FP64 reconstruction)",
+            launches_per_step=n, ms_per_step=ms, avg_launch_us=1e3 * ms / n, bound="mfma",
+            achieved=ops, peak=PEAK_INT8_TOPS, unit="TOP/s", frac=ops / PEAK_INT8_TOPS,
+            fp64_equivalent_tflops=ach, work_per_launch=work / n,
+            work=f"4*M*K flops per complex band mode, executed as {npair} int8 slice products "
+                 "(achieved counts int8 ops on the unpadded M x K; tile padding not counted)",
+            traffic=None)
     ms, n, work = prof["shell_solve"]
     if n == 0:
         return out   # shell solves ran as contractions in the eigenbasis of B0
@@ -150,6 +165,8 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=256, help="override Nz (other workloads)")
     ap.add_argument("--nx", type=int, default=256, help="override Nx = Ny (other workloads)")
     ap.add_argument("--dt", type=float, default=2.5e-5, help="override dt (other workloads)")
+    ap.add_argument("--mixed-slices", type=int, default=0,
+                    help="mixed-precision step with this many int8 slices (4..7); 0 = FP64")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -163,9 +180,10 @@ def main() -> int:
     from nhqg_amd.init import polar_config_256, polar_state_256
 
     cfg = polar_config_256(Nx=args.nx, Nz=args.nz, dt=args.dt)
-    headline = (args.nx, args.nz, args.dt) == (256, 256, 2.5e-5)
+    headline = (args.nx, args.nz, args.dt, args.mixed_slices) == (256, 256, 2.5e-5, 0)
     t0 = time.perf_counter()
     g = make_grid(cfg, shells="band")
+    g.mixed_slices = args.mixed_slices
     t_grid = time.perf_counter() - t0
     st0 = polar_state_256(g, seed=0)
     t0 = time.perf_counter()
@@ -186,6 +204,9 @@ def main() -> int:
     sp.sync()
     wall = time.perf_counter() - t0
     dev_ms = sp.last_ms()
+    # state after warmup + timed steps (the profiled step below advances one more)
+    final = sp.state() if args.mixed_slices else None
+    dmix = sp.diagnostics() if args.mixed_slices else None
     diag = sp.diagnostics()
     prof = sp.ctx.profile_step()
     # plain 1 GiB device copy: practical HBM ceiling and the known-size
@@ -225,13 +246,30 @@ def main() -> int:
         "device_bytes": int(sp.ctx.device_bytes()),
         "diagnostics_after": {k: diag[k] for k in ("KE_tot", "max_speed", "Nusselt_dealiased")},
     }
-    rl = rooflines(prof, g, lay, full_size=headline)
+    rl = rooflines(prof, g, lay, full_size=headline, slices=args.mixed_slices)
     dom = max(rl, key=lambda k: rl[k]["ms_per_step"])
     res["roofline"] = dict(rl[dom], kernel_class=dom)
     res["rooflines"] = rl
     res["roofline_note"] = ("traffic: HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/"
                             "WRITE_SIZE summary (" + os.path.relpath(TRAFFIC_FILE, ROOT) + "), "
                             "calibrated on a 1 GiB copy")
+    if args.mixed_slices:
+        # the same warmup + timed steps on the plain FP64 path from the same state
+        g0 = make_grid(cfg, shells="band")
+        sp0 = Stepper(g0)
+        sp0.set_state(st0)
+        sp0.run(args.warmup + args.steps)
+        ref = sp0.state()
+        d0 = sp0.diagnostics()
+        err = {k: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+               for k, a, b in zip(("q_hat", "w_hat", "th_hat", "th_bar"), final, ref)}
+        err.update({k: abs(dmix[k] - d0[k]) / abs(d0[k])
+                    for k in ("KE_tot", "Nusselt_dealiased", "max_speed")})
+        res["mixed"] = {"slices": args.mixed_slices, "steps_compared": args.warmup + args.steps,
+                        "rel_error_vs_fp64_gpu": err}
+        res["dtype"] = f"f64 (contractions as int8 x{args.mixed_slices} slices, int32 exact)"
+        res["config"]["workload"] = "mixed-precision " + res["config"]["workload"]
+        g0.close()
     if not args.no_cpu:
         base, par = cpu_baseline(cfg, st0, args.cpu_steps, one)
         res["cpu_baseline"] = base

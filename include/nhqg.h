@@ -49,7 +49,12 @@ typedef struct {
   int32_t advection;   /* 0: flux form (the production branch) */
   int32_t w_neumann;   /* 1: w in the mixed bottom-Dirichlet/top-Neumann basis */
   int32_t apply_mask;  /* 1: layout keeps out-of-band modes; mask outputs */
-  int32_t reserved;
+  int32_t mixed_slices; /* 0: plain FP64 step.  4..7: mixed-precision step (no reference
+                         * counterpart): the Chebyshev transforms and implicit-solve maps
+                         * run as int8 slice products with exact int32 accumulation and
+                         * FP64 reconstruction, this many 7-bit slices per operand; the
+                         * SBP exchange, the k = 0 equation, FFT products and
+                         * diagnostics stay FP64 */
   double dt, gamma_imex, ra_sigma, sigma, eps_sq, beta;
 } nhqg_params;
 

@@ -80,6 +80,10 @@ struct nhqg_ctx {
   bool cl_on = false;
   int cl_n0 = 0;
   float cl_in = 1.0f, cl_out = 1.0f;
+  // mixed precision: int8 slices per operand (0 = FP64); mixed_now is set
+  // while the step's contractions are enqueued (diagnostics stay FP64)
+  int mixed_slices = 0;
+  bool mixed_now = false;
 };
 
 namespace {
@@ -136,6 +140,27 @@ int upload_op(nhqg_ctx* c, GemmOp* op, const double* host, int M, int K, int mt)
   return r;
 }
 
+// int8 slices of an uploaded operator for the mixed-precision kernel
+int upload_op_i8(nhqg_ctx* c, GemmOp* op, const double* host, int S) {
+  if (!host || S == 0 || op->K < 2) return NHQG_OK;
+  std::vector<int8_t> a8;
+  std::vector<double> sa, al;
+  gemm_i8_split(host, op->M, op->K, S, a8, sa, al);
+  int8_t* d8 = nullptr;
+  double *dsa = nullptr, *dal = nullptr;
+  int r = upload(c, &d8, a8.data(), a8.size());
+  if (!r) r = upload(c, &dsa, sa.data(), sa.size());
+  if (!r) r = upload(c, &dal, al.data(), al.size());
+  if (r) return r;
+  op->A8 = d8;
+  op->sa = dsa;
+  op->afirst = dal;
+  op->slices = S;
+  op->nti = (op->M + 15) / 16;
+  op->nk = (op->K - 1 + 63) / 64;
+  return NHQG_OK;
+}
+
 int field(nhqg_ctx* c, double** p, int nl) {
   int r = dalloc(c, p, (size_t)nl * c->ld);
   if (r) return r;
@@ -167,11 +192,12 @@ enum {
   CL_SBP,
   CL_ELEMENTWISE,
   CL_SMALL,
+  CL_GEMM_I8,
   CL_COUNT
 };
 const char* const kClassNames[CL_COUNT] = {
     "gemm_f64_mfma", "flux_col_inverse", "flux_row_products", "flux_col_forward",
-    "shell_solve", "parseval_means", "sbp_serial", "elementwise", "small"};
+    "shell_solve", "parseval_means", "sbp_serial", "elementwise", "small", "gemm_i8_mixed"};
 
 void prof_begin(nhqg_ctx* c, hipEvent_t* a) {
   (void)hipEventCreate(a);
@@ -230,7 +256,16 @@ int gemm_jobs(nhqg_ctx* c, std::initializer_list<Job> jobs) {
     g.e2 = j.e2;
     work += 4.0 * j.op->M * j.op->K * (double)c->nm;
   }
-  PL(CL_GEMM, work, launch_gemm_batch(b, n, c->st));
+  // int8 slice kernel inside the step for operators that carry slices
+  // (not the SBP transfers, not the fused substep update)
+  bool i8 = c->mixed_now && c->mixed_slices > 0;
+  for (int k = 0; k < n && i8; ++k) i8 = b.job[k].op.A8 != nullptr && b.job[k].W == nullptr;
+  if (i8) {
+    b.slices = c->mixed_slices;
+    PL(CL_GEMM_I8, work, launch_gemm_i8(b, n, c->st));
+  } else {
+    PL(CL_GEMM, work, launch_gemm_batch(b, n, c->st));
+  }
   return NHQG_OK;
 }
 
@@ -425,7 +460,7 @@ int sanitize(nhqg_ctx* c) {
 }
 
 // One full step y <- imex_step(y).
-int enqueue_step(nhqg_ctx* c) {
+int enqueue_step_body(nhqg_ctx* c) {
   const Ops& o = c->o;
   const double dt = c->P.dt;
   const double g = c->P.gamma_imex;
@@ -513,6 +548,16 @@ int enqueue_step(nhqg_ctx* c) {
   return NHQG_OK;
 }
 
+// the step's contractions take the int8 slice kernel when mixed precision
+// is on; everything else (diagnostics, component calls) stays FP64 unless
+// it sets mixed_now itself
+int enqueue_step(nhqg_ctx* c) {
+  c->mixed_now = c->mixed_slices > 0;
+  const int r = enqueue_step_body(c);
+  c->mixed_now = false;
+  return r;
+}
+
 int build_graph(nhqg_ctx* c) {
   if (c->gexec) return NHQG_OK;
   CK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
@@ -549,6 +594,10 @@ int check_params(const nhqg_params* p, std::string& err) {
   if (p->advection != 0) {
     err = "only the flux advection form is built (advection=0)";
     return NHQG_ERR_UNSUPPORTED;
+  }
+  if (p->mixed_slices != 0 && !gemm_i8_supported(p->mixed_slices)) {
+    err = "mixed_slices must be 0 (FP64) or in [4, 7]";
+    return NHQG_ERR_ARG;
   }
   if (p->n_shells < 1) {
     err = "n_shells must be >= 1";
@@ -616,6 +665,25 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
     RUN(upload_op(c, &o.Mwt, h->Mwt, Ng, Ng, mt));
     RUN(upload_op(c, &o.Mth2w, h->Mth2w, Ng, Ng, mt));
   }
+  // int8 slices of the Chebyshev transforms and the implicit-solve maps
+  // (the SBP transfers, Psbp and the diagnostics operators stay FP64)
+  c->mixed_slices = p->mixed_slices;
+  {
+    const int S = c->mixed_slices;
+    RUN(upload_op_i8(c, &o.V, h->V, S));
+    RUN(upload_op_i8(c, &o.VSw, h->VSw, S));
+    RUN(upload_op_i8(c, &o.VSd, h->VSd, S));
+    RUN(upload_op_i8(c, &o.Vinv, h->Vinv, S));
+    RUN(upload_op_i8(c, &o.Pw, h->Pw, S));
+    RUN(upload_op_i8(c, &o.Pd, h->Pd, S));
+    RUN(upload_op_i8(c, &o.Qw, h->Qw, S));
+    RUN(upload_op_i8(c, &o.GSw, h->GSw, S));
+    if (p->w_neumann) {
+      RUN(upload_op_i8(c, &o.Mtw, h->Mtw, S));
+      RUN(upload_op_i8(c, &o.Mwt, h->Mwt, S));
+      RUN(upload_op_i8(c, &o.Mth2w, h->Mth2w, S));
+    }
+  }
   RUN(upload(c, &o.rows, h->rows, (size_t)c->nkx));
   RUN(upload(c, &o.kidx_of_x, h->kidx_of_x, (size_t)c->Nx));
   RUN(upload(c, &o.full_idx, h->full_idx, (size_t)nm));
@@ -654,6 +722,8 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   if (eigen) {
     RUN(upload_op(c, &o.eigQ, h->eig_Q, Ng, Ng, mt));
     RUN(upload_op(c, &o.eigQinv, h->eig_Qinv, Ng, Ng, mt));
+    RUN(upload_op_i8(c, &o.eigQ, h->eig_Q, c->mixed_slices));
+    RUN(upload_op_i8(c, &o.eigQinv, h->eig_Qinv, c->mixed_slices));
     RUN(upload(c, &o.lam, h->eig_lam, (size_t)Ng));
     // padding modes: alpha 1, c 0 (their columns are zero)
     std::vector<double> a((size_t)c->nm_pad, 1.0), cm((size_t)c->nm_pad, 0.0);
@@ -1022,9 +1092,18 @@ int nhqg_diagnostics(nhqg_ctx* c, double* out, int32_t n_out) {
   return diagnostics_impl(c, out, n_out);
 }
 
+static int apply_impl(nhqg_ctx* c, int32_t op, double arg);
+
 int nhqg_apply(nhqg_ctx* c, int32_t op, double arg) {
   if (!c) return NHQG_ERR_ARG;
   if (!c->have_state) return NHQG_ERR_STATE;
+  c->mixed_now = c->mixed_slices > 0;
+  const int r = apply_impl(c, op, arg);
+  c->mixed_now = false;
+  return r;
+}
+
+static int apply_impl(nhqg_ctx* c, int32_t op, double arg) {
   switch (op) {
     case NHQG_OP_EXPLICIT_RHS:
       RUN(explicit_rhs(c, c->yq, c->yw, c->yth, c->E1q, c->E1w, c->E1th));

@@ -13,6 +13,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
+#include <vector>
 
 #define NHQG_DEV __device__ __forceinline__
 
@@ -148,6 +149,13 @@ hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mo
 struct GemmOp {
   const double* At;
   int M, K, mt, lda, kp;
+  // int8 slices for the mixed-precision kernel (gemm_i8.hip), or null:
+  // A8 in tile order, sa[r] = 2^ea_r, afirst[r] = A[r][0] (k = 0 runs in
+  // FP64); nti row tiles of 16, nk k-steps of 64 over k = 1 .. K-1
+  const void* A8 = nullptr;
+  const double* sa = nullptr;
+  const double* afirst = nullptr;
+  int slices = 0, nti = 0, nk = 0;
 };
 // One contraction C = epilogue(A * B') of a batch.  B' = B, or with W set
 // the SBP substep update B - bc * W * g[row] with rows 0 and K-1 zeroed.
@@ -165,7 +173,8 @@ struct GemmJob {
 #define GEMM_MAX_JOBS 3
 struct GemmBatch {
   GemmJob job[GEMM_MAX_JOBS];
-  int ld;   // row stride (doubles) of every B, C, W and epilogue field
+  int ld;       // row stride (doubles) of every B, C, W and epilogue field
+  int slices;   // 0: FP64 MFMA kernel; 4..7: int8 slice kernel (every op split)
 };
 int gemm_row_tiles(int M);
 int gemm_op_lda(int mt);
@@ -174,6 +183,12 @@ void gemm_pack_op(const double* A, int M, int K, int mt, double* out);
 // Runs b.job[0..nj) in one launch; every job has the same mt and all or
 // none use the update.  ld must be a multiple of 128.
 hipError_t launch_gemm_batch(const GemmBatch& b, int nj, hipStream_t s);
+// Mixed precision (gemm_i8.hip): the same contract from int8 slice products,
+// b.slices slices per operand; no fused SBP update.
+bool gemm_i8_supported(int S);
+hipError_t launch_gemm_i8(const GemmBatch& b, int nj, hipStream_t s);
+void gemm_i8_split(const double* A, int M, int K, int S, std::vector<int8_t>& a8,
+                   std::vector<double>& sa, std::vector<double>& afirst);
 // Learned closure (closure.hip): E_q[n0 + j][c] += out_scale * MLP(in_scale *
 // q[n0 .. n0+15][c]) for every column c < nm2 of the level-major fields.
 hipError_t launch_closure(int nm2, int ld, int n0, const void* wb, const double* q, double* Eq,

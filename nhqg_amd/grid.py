@@ -299,6 +299,10 @@ class Grid:
     # (two dense contractions, no per-shell matrices); 'matrices' streams
     # the per-shell inverses like the reference
     shell_solver: str = "eigen"
+    # 0: plain FP64 step; 4..7: mixed-precision step with this many int8
+    # slices per operand in the Chebyshev transforms and implicit-solve maps
+    # (set before the first context is created)
+    mixed_slices: int = 0
     _layouts: dict = field(default_factory=dict, repr=False)
     _contexts: dict = field(default_factory=dict, repr=False)
 
