@@ -155,6 +155,44 @@ def cpu_baseline(cfg, st, steps: int, gpu_one_step):
     return base, par
 
 
+def mixed_leg(cfg, st0, args, ref, dref) -> dict:
+    """The mixed-precision step (int8 slice contractions) on the same state,
+    warmup and timed steps as the plain run; errors against the plain run's
+    state after the same number of steps."""
+    import numpy as np
+
+    from nhqg_amd import Stepper, make_grid
+
+    g = make_grid(cfg, shells="band")
+    g.mixed_slices = args.mixed_slices
+    sp = Stepper(g)
+    sp.set_state(st0)
+    sp.run(args.warmup)
+    sp.sync()
+    t0 = time.perf_counter()
+    sp.run_async(args.steps)
+    sp.sync()
+    wall = time.perf_counter() - t0
+    fin = sp.state()
+    d = sp.diagnostics()
+    err = {k: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+           for k, a, b in zip(("q_hat", "w_hat", "th_hat", "th_bar"), fin, ref)}
+    err.update({k: abs(d[k] - dref[k]) / abs(dref[k])
+                for k in ("KE_tot", "Nusselt_dealiased", "max_speed")})
+    prof = sp.ctx.profile_step()
+    rl = rooflines(prof, g, sp.ctx.layout, full_size=False, slices=args.mixed_slices)
+    out = {"value": args.steps / wall, "unit": "steps/s", "ms_per_step": 1e3 * wall / args.steps,
+           "slices": args.mixed_slices,
+           "dtype": f"f64 state; contractions as int8 x{args.mixed_slices} slice products "
+                    "(int32 exact) with FP64 reconstruction",
+           "steps_compared": args.warmup + args.steps,
+           "rel_error_vs_plain_fp64": err,
+           "kernels": {k: {"ms_per_step": v[0], "launches": v[1]} for k, v in prof.items()},
+           "roofline": rl.get("gemm_i8_mixed")}
+    g.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,8 +203,9 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=256, help="override Nz (other workloads)")
     ap.add_argument("--nx", type=int, default=256, help="override Nx = Ny (other workloads)")
     ap.add_argument("--dt", type=float, default=2.5e-5, help="override dt (other workloads)")
-    ap.add_argument("--mixed-slices", type=int, default=0,
-                    help="mixed-precision step with this many int8 slices (4..7); 0 = FP64")
+    ap.add_argument("--mixed-slices", type=int, default=6,
+                    help="also time the mixed-precision step with this many int8 slices (4..7) "
+                         "and its error against the plain run; 0 skips it")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -180,10 +219,9 @@ def main() -> int:
     from nhqg_amd.init import polar_config_256, polar_state_256
 
     cfg = polar_config_256(Nx=args.nx, Nz=args.nz, dt=args.dt)
-    headline = (args.nx, args.nz, args.dt, args.mixed_slices) == (256, 256, 2.5e-5, 0)
+    headline = (args.nx, args.nz, args.dt) == (256, 256, 2.5e-5)
     t0 = time.perf_counter()
     g = make_grid(cfg, shells="band")
-    g.mixed_slices = args.mixed_slices
     t_grid = time.perf_counter() - t0
     st0 = polar_state_256(g, seed=0)
     t0 = time.perf_counter()
@@ -206,7 +244,6 @@ def main() -> int:
     dev_ms = sp.last_ms()
     # state after warmup + timed steps (the profiled step below advances one more)
     final = sp.state() if args.mixed_slices else None
-    dmix = sp.diagnostics() if args.mixed_slices else None
     diag = sp.diagnostics()
     prof = sp.ctx.profile_step()
     # plain 1 GiB device copy: practical HBM ceiling and the known-size
@@ -246,7 +283,7 @@ def main() -> int:
         "device_bytes": int(sp.ctx.device_bytes()),
         "diagnostics_after": {k: diag[k] for k in ("KE_tot", "max_speed", "Nusselt_dealiased")},
     }
-    rl = rooflines(prof, g, lay, full_size=headline, slices=args.mixed_slices)
+    rl = rooflines(prof, g, lay, full_size=headline)
     dom = max(rl, key=lambda k: rl[k]["ms_per_step"])
     res["roofline"] = dict(rl[dom], kernel_class=dom)
     res["rooflines"] = rl
@@ -254,22 +291,7 @@ def main() -> int:
                             "WRITE_SIZE summary (" + os.path.relpath(TRAFFIC_FILE, ROOT) + "), "
                             "calibrated on a 1 GiB copy")
     if args.mixed_slices:
-        # the same warmup + timed steps on the plain FP64 path from the same state
-        g0 = make_grid(cfg, shells="band")
-        sp0 = Stepper(g0)
-        sp0.set_state(st0)
-        sp0.run(args.warmup + args.steps)
-        ref = sp0.state()
-        d0 = sp0.diagnostics()
-        err = {k: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
-               for k, a, b in zip(("q_hat", "w_hat", "th_hat", "th_bar"), final, ref)}
-        err.update({k: abs(dmix[k] - d0[k]) / abs(d0[k])
-                    for k in ("KE_tot", "Nusselt_dealiased", "max_speed")})
-        res["mixed"] = {"slices": args.mixed_slices, "steps_compared": args.warmup + args.steps,
-                        "rel_error_vs_fp64_gpu": err}
-        res["dtype"] = f"f64 (contractions as int8 x{args.mixed_slices} slices, int32 exact)"
-        res["config"]["workload"] = "mixed-precision " + res["config"]["workload"]
-        g0.close()
+        res["mixed"] = mixed_leg(cfg, st0, args, final, diag)
     if not args.no_cpu:
         base, par = cpu_baseline(cfg, st0, args.cpu_steps, one)
         res["cpu_baseline"] = base

@@ -46,7 +46,8 @@ typedef struct {
   int32_t n_pairs;     /* entries of herm_pairs */
   int32_t substeps;    /* SBP corrector substeps (sbp_corrector_substeps) */
   int32_t evolve_mean; /* 1: evolve_mean + balanced_sbp2_pc corrector */
-  int32_t advection;   /* 0: flux form (the production branch) */
+  int32_t advection;   /* 0: flux form (the production branch), 1: Jacobian form
+                        * (nonlinear_advection = "jacobian", nhqg/spectral.py:281-304) */
   int32_t w_neumann;   /* 1: w in the mixed bottom-Dirichlet/top-Neumann basis */
   int32_t apply_mask;  /* 1: layout keeps out-of-band modes; mask outputs */
   int32_t mixed_slices; /* 0: plain FP64 step.  4..7: mixed-precision step (no reference
@@ -120,6 +121,9 @@ typedef struct {
   const double *eig_lam;      /* Ng eigenvalues */
   const double *mode_alpha;   /* per mode: alpha of its shell */
   const double *mode_c;       /* per mode: c of its shell */
+  /* Jacobian form with a trap: d(eta)/dx, d(eta)/dy on the grid (Nx x Nx,
+   * row x, column y), the gradient J(psi, q + eta) adds to q's */
+  const double *eta_x_phys, *eta_y_phys;
 } nhqg_operators;
 
 /* Diagnostics vector layout returned by nhqg_diagnostics (length

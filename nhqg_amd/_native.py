@@ -51,7 +51,7 @@ _OPS_FIELDS = [
     ("herm_pairs", _ip), ("shell_ptr", _ip), ("shell_modes", _ip), ("shell_inv", _dp),
     ("eta_phys", _dp), ("sponge_phys", _dp), ("twiddle", _dp), ("shell_B0", _dp),
     ("shell_K", _dp), ("shell_coef", _dp), ("eig_Q", _dp), ("eig_Qinv", _dp), ("eig_lam", _dp),
-    ("mode_alpha", _dp), ("mode_c", _dp),
+    ("mode_alpha", _dp), ("mode_c", _dp), ("eta_x_phys", _dp), ("eta_y_phys", _dp),
 ]
 
 
@@ -166,7 +166,7 @@ class Context:
         p.n_pairs = int(t["herm_pairs"].size // 2)
         p.substeps = max(1, int(grid.sbp_corrector_substeps))
         p.evolve_mean = 1 if grid.evolve_mean else 0
-        p.advection = 0
+        p.advection = 1 if grid.nonlinear_advection == "jacobian" else 0
         p.w_neumann = 1 if grid.w_bc_top == "neumann" else 0
         p.apply_mask = 1 if layout.full else 0
         p.mixed_slices = int(getattr(grid, "mixed_slices", 0))

@@ -63,10 +63,22 @@ class PairSet:
 
 
 def make_pairs(gf: Grid, gc: Grid, state0: State, spinup: int, snapshots: int, every: int,
-               seed: int = 0, n_samples: int | None = None) -> PairSet:
+               seed: int = 0, n_samples: int | None = None, horizon: int = 1) -> PairSet:
     """Fine run from state0: after `spinup` steps, `snapshots` snapshots
-    `every` steps apart; at each, one fine and one coarse step from the
-    filtered state give a pair set; optionally subsampled to n_samples."""
+    `every` steps apart; optionally subsampled to n_samples.
+
+    horizon = 1: at each snapshot, one fine and one coarse step from the
+    filtered state; x = the filtered state's top q levels, target = the
+    one-step missing tendency.
+    horizon = n > 1: the coarse model runs n steps from the filtered
+    snapshot; x = the coarse state's top q levels after those n steps (the
+    states the closure meets in a coarse run, tail pile-up included),
+    target = (filtered fine state n steps on - coarse state) / (n dt), the
+    mean tendency that would have kept the coarse run on the filtered fine
+    trajectory.  The fine run then advances `every` steps (every >= n)."""
+    if horizon > 1:
+        return _relaxation_pairs(gf, gc, state0, spinup, snapshots, every, seed, n_samples,
+                                 horizon)
     n0 = gc.Nz + 1 - N_TOP
     fine = Stepper(gf)
     fine.set_state(state0)
@@ -97,6 +109,36 @@ def make_pairs(gf: Grid, gc: Grid, state0: State, spinup: int, snapshots: int, e
         pick = np.random.default_rng(seed).choice(x.shape[0], n_samples, replace=False)
         x, t = x[pick], t[pick]
     return PairSet(x, t, float(gf.dt), n0)
+
+
+def _relaxation_pairs(gf, gc, state0, spinup, snapshots, every, seed, n_samples, horizon):
+    n0 = gc.Nz + 1 - N_TOP
+    if every < horizon:
+        raise ValueError("every must be >= horizon")
+    fine = Stepper(gf)
+    fine.set_state(state0)
+    if spinup:
+        fine.run(spinup)
+    coarse = Stepper(gc)
+    xs, ts = [], []
+    for _ in range(snapshots):
+        sc = filter_state(fine.state(), gf, gc)
+        coarse.set_state(sc)
+        coarse.run(horizon)
+        qc = coarse.state().q_hat
+        fine.run(horizon)
+        qf = filter_state(fine.state(), gf, gc).q_hat
+        if every > horizon:
+            fine.run(every - horizon)
+        r = (qf[n0:] - qc[n0:]) / (horizon * float(gc.dt))
+        xs.append(band_columns(qc[n0:], gc.mask_23))
+        ts.append(band_columns(r, gc.mask_23))
+    x = np.concatenate(xs)
+    t = np.concatenate(ts)
+    if n_samples is not None and n_samples < x.shape[0]:
+        pick = np.random.default_rng(seed).choice(x.shape[0], n_samples, replace=False)
+        x, t = x[pick], t[pick]
+    return PairSet(x, t, float(gc.dt), n0)
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -281,6 +323,8 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed-train", type=int, default=0)
     ap.add_argument("--seed-judge", type=int, default=10)
+    ap.add_argument("--horizon", type=int, default=400,
+                    help="1: one-step pairs; n > 1: n-step relaxation pairs from coarse runs")
     args = ap.parse_args(argv)
 
     cfg_f = polar_config_256(Nx=args.nx, Nz=args.nz_fine)
@@ -294,7 +338,7 @@ def main(argv=None) -> int:
     gf, gc = make_grid(cfg_f, shells="band"), make_grid(cfg_c, shells="band")
     pairs = make_pairs(gf, gc, seeded(gf, args.seed_train), spinup=args.spinup,
                        snapshots=args.snapshots, every=args.snap_every, seed=args.seed_train,
-                       n_samples=args.pairs)
+                       n_samples=args.pairs, horizon=args.horizon)
     t_pairs = time.perf_counter() - t0
     print(f"pairs {pairs.x.shape[0]} in {t_pairs:.1f} s", flush=True)
     gf.close()

@@ -32,7 +32,7 @@ struct Ops {
   int2* pairs;
   double *kx_row, *ky_col, *kx_mode, *ksq, *inv_denom, *iaq, *iath, *idq, *diss_q, *diss_w,
       *diss_th, *mask, *pw, *ones, *pw_ke, *pw_cc;
-  double *shell_inv, *eta_phys, *sponge_phys;
+  double *shell_inv, *eta_phys, *sponge_phys, *eta_x, *eta_y;
   double2* tw;
 };
 
@@ -318,6 +318,9 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   a.mask = o.mask;
   a.eta_phys = o.eta_phys;
   a.sponge_phys = o.sponge_phys;
+  a.jac = c->P.advection;
+  a.eta_x = o.eta_x;
+  a.eta_y = o.eta_y;
   a.tw = o.tw;
   a.X5 = c->X5;
   a.Y6 = c->Y6;
@@ -325,9 +328,11 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   const double F = 16.0 * c->Nq * (double)c->Nx * c->nky;   // one intermediate field
   const double Fc = 16.0 * c->Nq * (double)c->nm;            // one compact field
   const int sp = a.sponge_phys ? 3 : 0;
-  PL(CL_FLUX_COLINV, (3 + sp) * Fc + (5 + sp) * F, launch_flux(a, 1, c->st));
-  PL(CL_FLUX_ROW, (11 + 2 * sp) * F, launch_flux(a, 2, c->st));
-  PL(CL_FLUX_COLFWD, (6 + sp) * F + (3 + sp) * Fc, launch_flux(a, 3, c->st));
+  // intermediates per pass: flux form 5 in / 6 out, Jacobian form 8 in / 3 out
+  const int nxin = a.jac ? 8 : 5, nyout = a.jac ? 3 : 6;
+  PL(CL_FLUX_COLINV, (3 + sp) * Fc + (nxin + sp) * F, launch_flux(a, 1, c->st));
+  PL(CL_FLUX_ROW, (nxin + nyout + 2 * sp) * F, launch_flux(a, 2, c->st));
+  PL(CL_FLUX_COLFWD, (nyout + sp) * F + (3 + sp) * Fc, launch_flux(a, 3, c->st));
   // E = -(back transform of the advection) - sponge (solver.py:541-549, 595-597)
   Epi eq = epi_make(-1.0), ew = epi_make(-1.0), et = epi_make(-1.0);
   if (a.sponge_phys) {
@@ -591,8 +596,8 @@ int check_params(const nhqg_params* p, std::string& err) {
     err = "mode layout out of range";
     return NHQG_ERR_ARG;
   }
-  if (p->advection != 0) {
-    err = "only the flux advection form is built (advection=0)";
+  if (p->advection != 0 && p->advection != 1) {
+    err = "advection must be 0 (flux form) or 1 (Jacobian form)";
     return NHQG_ERR_UNSUPPORTED;
   }
   if (p->mixed_slices != 0 && !gemm_i8_supported(p->mixed_slices)) {
@@ -783,6 +788,16 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
     }
   }
   RUN(upload(c, &o.eta_phys, h->eta_phys, (size_t)c->Nx * c->Nx));
+  if (p->advection == 1 && h->eta_phys) {
+    if (!h->eta_x_phys || !h->eta_y_phys) {
+      c->err = "the Jacobian form with a trap needs eta_x_phys and eta_y_phys";
+      return NHQG_ERR_ARG;
+    }
+    RUN(upload(c, &o.eta_x, h->eta_x_phys, (size_t)c->Nx * c->Nx));
+    RUN(upload(c, &o.eta_y, h->eta_y_phys, (size_t)c->Nx * c->Nx));
+  } else {
+    o.eta_x = o.eta_y = nullptr;
+  }
   RUN(upload(c, &o.sponge_phys, h->sponge_phys, (size_t)c->Nx * c->Nx));
   RUN(upload(c, (double**)&o.tw, h->twiddle, (size_t)2 * c->Nx));
 
@@ -821,9 +836,11 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(dalloc(c, &c->partial, (size_t)Nq * c->Pred + 64));
   RUN(dalloc(c, &c->sbp, (size_t)Nq * 16));
   RUN(dalloc(c, &c->g, (size_t)Nq));
+  // FFT intermediates, one block of 17 fields: flux form 8 in + 9 out,
+  // Jacobian form 11 in + 6 out
   c->fstride = (size_t)Nq * c->Nx * c->nky;
-  RUN(dalloc(c, &c->X5, c->fstride * 8));
-  RUN(dalloc(c, &c->Y6, c->fstride * 9));
+  RUN(dalloc(c, &c->X5, c->fstride * 17));
+  c->Y6 = c->X5 + c->fstride * (p->advection ? 11 : 8);
   RUN(dalloc(c, &c->staging, (size_t)Nq * c->Nx * c->Nk));
   RUN(dalloc(c, &c->diag, (size_t)64 + 8 * Nq));
   RUN(dalloc(c, &c->rowmax, (size_t)Nq * c->Nx));

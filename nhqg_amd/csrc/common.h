@@ -95,6 +95,8 @@ struct FluxArgs {
   double2 *sq, *sw, *sth;          // sponge outputs (coefficient levels)
   const int *kidx_of_x, *rows;
   const double *kx_row, *ky_col, *inv_denom, *mask, *eta_phys, *sponge_phys;
+  int jac;                        // 0 flux form, 1 Jacobian form
+  const double *eta_x, *eta_y;    // grad(eta) on the grid (Jacobian form with a trap)
   const double2* tw;
   double2 *X5, *Y6;
   size_t fstride;

@@ -104,11 +104,14 @@ NHQG_DEV void load_twiddles(double2* tw, const double2* twg) {
 }
 
 // ---------------------------------------------------------------------------
-// Pass 1: inverse DFT along x of the five advected inputs.
+// Pass 1: inverse DFT along x of the advected inputs.
 // qn/wn/thn: nodal spectral fields (level-major, row stride ldc double2).
-// X5[f][l][x][ky], f = U, V, Q, W, Theta.
+// Flux form: X5[f][l][x][ky], f = U, V, Q, W, Theta (+ sponge q, w, theta
+// coefficient levels).  Jacobian form: U, V, then per field f and i kx f
+// (Q, iKx Q, W, iKx W, Theta, iKx Theta; i ky is applied along y in pass 2),
+// then the sponge inputs.
 // ---------------------------------------------------------------------------
-template <int N, bool SPONGE>
+template <int N, bool SPONGE, bool JAC>
 __global__ void __launch_bounds__(256)
 col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ qn,
                  const double2* __restrict__ wn, const double2* __restrict__ thn,
@@ -118,7 +121,8 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
                  const double* __restrict__ inv_denom, const double2* __restrict__ twg,
                  double2* __restrict__ X5, size_t fstride) {
   constexpr int T = N / 4, TEAMS = 256 / T;
-  constexpr int NF = SPONGE ? 8 : 5;
+  constexpr int NA = JAC ? 8 : 5;   // advection inputs
+  constexpr int NF = NA + (SPONGE ? 3 : 0);
   __shared__ double2 buf[2][TEAMS][N];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
@@ -130,21 +134,22 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
   const size_t lb = (size_t)l * ldc;
   for (int f = 0; f < NF; ++f) {
     // w and theta coefficient fields have ng < Nz+1 levels (block-uniform)
-    if (f >= 6 && l >= ng) continue;
+    if (f >= NA + 1 && l >= ng) continue;
     for (int x = t; x < N; x += T) {
       double2 v = make_double2(0.0, 0.0);
       const int i = kidx_of_x[x];
       if (active && i >= 0) {
         const int m = ky * nkx + i;
-        if (f < 3) {
+        if (f < 2) {
           const double2 qv = qn[lb + m];
           if (f == 0) v = cmul_i(qv, ky_col[ky] * inv_denom[m]);
-          else if (f == 1) v = cmul_i(qv, -kx_row[i] * inv_denom[m]);
-          else v = qv;
-        } else if (f < 5) {
-          v = (f == 3 ? wn : thn)[lb + m];
+          else v = cmul_i(qv, -kx_row[i] * inv_denom[m]);
+        } else if (f < NA) {
+          const int g = JAC ? (f - 2) / 2 : f - 2;   // advected field q, w, theta
+          v = (g == 0 ? qn : g == 1 ? wn : thn)[lb + m];
+          if (JAC && (f - 2) % 2 == 1) v = cmul_i(v, kx_row[i]);
         } else {
-          v = (f == 5 ? qc : f == 6 ? wc : thc)[lb + m];
+          v = (f == NA ? qc : f == NA + 1 ? wc : thc)[lb + m];
         }
       }
       buf[0][team][x] = v;
@@ -370,10 +375,111 @@ row_flux(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double
 }
 
 // ---------------------------------------------------------------------------
-// Pass 3: forward DFT along x; A_f(kx, ky) = i kx F_x(G_f) + F_x(H_f) at the
-// kept rows, times the output mask.  Writes A into the nodal buffers.
+// Pass 2, Jacobian form (nhqg/spectral.py:281-304): per (level, x) row,
+// f_y = F_y^-1(i ky Q_f) and f_x = F_y^-1(iKx Q_f) as one packed inverse,
+// J_f = u f_x + v f_y (= psi_x f_y - psi_y f_x), and G_f = F_y(J_f) into
+// Y[f]; sponge products into Y[3 + f].  The trap adds grad(eta) to q's
+// gradient (J(psi, q + eta)).
 // ---------------------------------------------------------------------------
-template <int N, bool SPONGE>
+template <int N, bool SPONGE, bool TRAP>
+__global__ void __launch_bounds__(256)
+row_jac(int nky, int ng, const double2* __restrict__ X, size_t fstride, double2* __restrict__ Y,
+        const double* __restrict__ ky_col, const double* __restrict__ eta_x,
+        const double* __restrict__ eta_y, const double* __restrict__ sponge_phys,
+        const double2* __restrict__ twg) {
+  constexpr int T = N / 4, TEAMS = 256 / T;
+  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 tw[N];
+  const int tid = threadIdx.x, team = tid / T, t = tid % T;
+  const int l = blockIdx.y;
+  int x = blockIdx.x * TEAMS + team;
+  const bool active = x < N;
+  if (!active) x = N - 1;  // duplicate a valid row; results are not stored
+  load_twiddles<N>(tw, twg);
+  __syncthreads();
+  double2* ba = buf[0][team];
+  double2* bb = buf[1][team];
+  const size_t ro = ((size_t)l * N + x) * nky;
+  const double sc = 1.0 / ((double)N * (double)N);
+  double ex[4], ey[4], sig[4], kyv[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    ex[j] = TRAP ? eta_x[(size_t)x * N + t + j * T] : 0.0;
+    ey[j] = TRAP ? eta_y[(size_t)x * N + t + j * T] : 0.0;
+    sig[j] = SPONGE ? sponge_phys[(size_t)x * N + t + j * T] : 0.0;
+    const int k = t + j * T;
+    const int kk = (k <= N / 2) ? k : N - k;
+    kyv[j] = kk < nky ? ky_col[kk] : 0.0;
+  }
+  Half4<N> hc, hn;
+  load_half<N>(hc, t, nky, X + 0 * fstride + ro, X + 1 * fstride + ro);
+  load_half<N>(hn, t, nky, X + 2 * fstride + ro, X + 3 * fstride + ro);
+  double u[4], v[4];
+  inv_pair_regs<N>(ba, bb, t, tw, hc, u, v, sc);
+#pragma unroll 1
+  for (int f = 0; f < 3; ++f) {
+    hc = hn;
+    if (f < 2)
+      load_half<N>(hn, t, nky, X + (4 + 2 * f) * fstride + ro, X + (5 + 2 * f) * fstride + ro);
+    else if (SPONGE)
+      load_half<N>(hn, t, nky, X + 8 * fstride + ro, nullptr);
+    // i ky on the half spectrum of f (entries stored at |ky|)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) hc.p[j] = cmul_i(hc.p[j], kyv[j]);
+    double fy[4], fx[4];
+    inv_pair_regs<N>(ba, bb, t, tw, hc, fy, fx, sc);
+    if (TRAP && f == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        fx[j] += ex[j];
+        fy[j] += ey[j];
+      }
+    }
+    double jv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) jv[j] = u[j] * fx[j] + v[j] * fy[j];
+    double2 G[3], dummy[3];
+    fwd_pair<N>(ba, bb, t, tw, jv, nullptr, nky, G, dummy);
+    if (active) {
+      double2* dg = Y + f * fstride + ro;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int ky = t + s * T;
+        if (ky < nky) dg[ky] = G[s];
+      }
+    }
+  }
+  if (SPONGE) {
+    // sigma times each coefficient level of q', w, theta (spectral.py:53-60)
+#pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+      if (f > 0 && l >= ng) break;  // block-uniform
+      hc = hn;
+      if (f < 2 && l < ng) load_half<N>(hn, t, nky, X + (9 + f) * fstride + ro, nullptr);
+      double c[4], a[4];
+      inv_pair_regs<N>(ba, bb, t, tw, hc, c, nullptr, sc);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[j] = sig[j] * c[j];
+      double2 S[3], dummy[3];
+      fwd_pair<N>(ba, bb, t, tw, a, nullptr, nky, S, dummy);
+      if (active) {
+        double2* ds = Y + (3 + f) * fstride + ro;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const int ky = t + s * T;
+          if (ky < nky) ds[ky] = S[s];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: forward DFT along x; A_f(kx, ky) = i kx F_x(G_f) + F_x(H_f) at the
+// kept rows (Jacobian form: A_f = F_x(G_f)), times the output mask.  Writes
+// A into the nodal buffers.
+// ---------------------------------------------------------------------------
+template <int N, bool SPONGE, bool JAC>
 __global__ void __launch_bounds__(256)
 col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ Y6,
                  size_t fstride, const int* __restrict__ rows, const double* __restrict__ kx_row,
@@ -392,7 +498,8 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
   load_twiddles<N>(tw, twg);
   double2* outs[3] = {out_q, out_w, out_th};
   double2* sps[3] = {sp_q, sp_w, sp_th};
-  constexpr int NG = SPONGE ? 3 : 2;  // G, H and (sponge) S per field
+  constexpr int NA = JAC ? 1 : 2;     // G (and H) per field
+  constexpr int NG = NA + (SPONGE ? 1 : 0);
   constexpr int NQ = 3 * NG;
   constexpr int NL = N * TEAMS / 256; // slot values gathered per thread
   // output rows, kx and mask of this thread's modes, the same for every field
@@ -407,10 +514,10 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
     mkv[j] = (in && active) ? mask[ky * nkx + i] : 0.0;
   }
   // the sponge slot of w and theta exists only on their ng levels
-  auto valid = [&](int q) { return !(q % NG == 2 && q / NG > 0 && l >= ng); };
+  auto valid = [&](int q) { return !(q % NG == NA && q / NG > 0 && l >= ng); };
   auto gather = [&](int q, double2 (&v)[NL]) {
     const int f = q / NG, h = q % NG;
-    const int slot = (h < 2) ? 2 * f + h : 6 + f;
+    const int slot = JAC ? (h == 0 ? f : 3 + f) : (h < 2) ? 2 * f + h : 6 + f;
     const double2* src = Y6 + slot * fstride + (size_t)l * N * nky;
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
@@ -444,7 +551,9 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
       if (i < nkx) {
         const double2 z = r[rowv[j]];
         const int m = ky * nkx + i;
-        if (h == 0) {
+        if (JAC) {
+          if (active) (h == 0 ? outs : sps)[f][(size_t)l * ldc + m] = cscale(z, mkv[j]);
+        } else if (h == 0) {
           g[j] = cmul_i(z, kxv[j]);
         } else if (h == 1) {
           if (active) outs[f][(size_t)l * ldc + m] = cscale(cadd(g[j], z), mkv[j]);
@@ -498,16 +607,23 @@ row_speed(int nky, const double2* __restrict__ X5, size_t fstride, double* __res
 // ---------------------------------------------------------------------------
 
 
-template <int N, bool SP>
+template <int N, bool SP, bool JAC>
 static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;
   dim3 gc((a.nky + TEAMS - 1) / TEAMS, a.nq);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
   if (pass == 0 || pass == 1)
-    hipLaunchKernelGGL((col_inverse_flux<N, SP>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.ng,
-                       a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row, a.ky_col,
+    hipLaunchKernelGGL((col_inverse_flux<N, SP, JAC>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc,
+                       a.ng, a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row, a.ky_col,
                        a.inv_denom, a.tw, a.X5, a.fstride);
-  if (pass == 0 || pass == 2) {
+  if ((pass == 0 || pass == 2) && JAC) {
+    if (a.eta_x)
+      hipLaunchKernelGGL((row_jac<N, SP, true>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
+                         a.fstride, a.Y6, a.ky_col, a.eta_x, a.eta_y, a.sponge_phys, a.tw);
+    else
+      hipLaunchKernelGGL((row_jac<N, SP, false>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
+                         a.fstride, a.Y6, a.ky_col, a.eta_x, a.eta_y, a.sponge_phys, a.tw);
+  } else if (pass == 0 || pass == 2) {
     if (a.eta_phys)
       hipLaunchKernelGGL((row_flux<N, SP, true>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
@@ -516,15 +632,20 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
                          a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
   }
   if (pass == 0 || pass == 3)
-    hipLaunchKernelGGL((col_forward_flux<N, SP>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, a.ng,
-                       a.Y6, a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath, a.sq, a.sw,
-                       a.sth, a.tw);
+    hipLaunchKernelGGL((col_forward_flux<N, SP, JAC>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc,
+                       a.ng, a.Y6, a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath, a.sq,
+                       a.sw, a.sth, a.tw);
 }
 
 template <int N>
 static hipError_t flux_n(const FluxArgs& a, int pass, hipStream_t s) {
-  if (a.sponge_phys) flux_pass<N, true>(a, pass, s);
-  else flux_pass<N, false>(a, pass, s);
+  if (a.jac) {
+    if (a.sponge_phys) flux_pass<N, true, true>(a, pass, s);
+    else flux_pass<N, false, true>(a, pass, s);
+  } else {
+    if (a.sponge_phys) flux_pass<N, true, false>(a, pass, s);
+    else flux_pass<N, false, false>(a, pass, s);
+  }
   return hipGetLastError();
 }
 
@@ -545,7 +666,7 @@ template <int N>
 static hipError_t speed_n(const FluxArgs& a, double* rowmax, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;
   dim3 gc((a.nky + TEAMS - 1) / TEAMS, a.nq);
-  hipLaunchKernelGGL((col_inverse_flux<N, false>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, 0,
+  hipLaunchKernelGGL((col_inverse_flux<N, false, false>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, 0,
                      a.qn, a.qn, a.qn, a.qn, a.qn, a.qn, a.kidx_of_x, a.kx_row, a.ky_col,
                      a.inv_denom, a.tw, a.X5, a.fstride);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);

@@ -9,7 +9,7 @@ context is created lazily on first use.
 
 Supported branch (anything else raises): float64, ``horizontal_dealiasing=
 '23_rule'``, ``vertical_dealiasing='none'``, ``q_boundary='none'``,
-``nonlinear_advection='flux'``, ``imex_scheme='ars222'``,
+``nonlinear_advection`` either ``'flux'`` or ``'jacobian'``, ``imex_scheme='ars222'``,
 ``sbp_transfer_mode='interp'``, ``thermal_closure`` either
 ``'fixed_conduction'`` or ``'evolve_mean'`` with
 ``mean_exchange_discretization='balanced_sbp2_pc'``, either ``w_bc_top``,
@@ -353,7 +353,6 @@ def _check_config(cfg: NHQGConfig) -> None:
         "horizontal_dealiasing": "23_rule",
         "vertical_dealiasing": "none",
         "q_boundary": "none",
-        "nonlinear_advection": "flux",
         "imex_scheme": "ars222",
         "sbp_transfer_mode": "interp",
     }
@@ -599,6 +598,11 @@ def device_operators(g: Grid, lay: ModeLayout) -> dict:
         ops.update(Mtw=Pw @ Sd, Mwt=g.map_w_to_th, Mth2w=g.map_th_to_w)
     Nx = g.Nx
     ops["eta_phys"] = None if g.eta_hat is None else np.fft.irfft2(g.eta_hat, s=(Nx, Nx))
+    if g.eta_hat is not None and g.nonlinear_advection == "jacobian":
+        # grad(eta) for J(psi, q + eta) (nhqg/solver.py:534-535 adds eta to q
+        # before the Jacobian, nhqg/spectral.py:293-298 differentiates)
+        ops["eta_x_phys"] = np.fft.irfft2(1j * g.kx * g.eta_hat, s=(Nx, Nx))
+        ops["eta_y_phys"] = np.fft.irfft2(1j * g.ky * g.eta_hat, s=(Nx, Nx))
     ops["sponge_phys"] = g.sponge_phys
     k = np.arange(Nx)
     ops["twiddle"] = np.stack([np.cos(2.0 * np.pi * k / Nx), -np.sin(2.0 * np.pi * k / Nx)], 1)

@@ -37,6 +37,8 @@ CONFIGS = {
     "neumann": cases.neumann_cfg,
     "fixed": lambda: cases.fixture_cfg(thermal_closure="fixed_conduction",
                                        mean_exchange_discretization="legacy"),
+    "jacobian": lambda: cases.fixture_cfg(nonlinear_advection="jacobian"),
+    "jacobian_trap_sponge": lambda: cases.trap_sponge_cfg(nonlinear_advection="jacobian"),
 }
 
 
