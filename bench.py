@@ -34,27 +34,30 @@ PEAK_INT8_TOPS = 5000.0   # dense int8 MFMA
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_polar_256x256x256.md")
+MIXED_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mixed_step_polar_256x256x256.md")
 # kernel-class name (nhqg_profile_step) -> HIP kernels it launches
 CLASS_KERNELS = {
     "gemm_f64_mfma": ("gemm_f64_kernel",),
+    "gemm_i8_mixed": ("gemm_i8_kernel",),
     "flux": ("col_inverse_flux", "row_flux", "col_forward_flux"),
     "shell_solve": ("shell_solve_kernel",),
     "parseval_means": ("parseval_partial_kernel",),
 }
 
 
-def profiled_traffic(cls: str):
-    """HBM bytes per launch of a kernel class from the committed rocprofv3
+def profiled_traffic(cls: str, path: str = TRAFFIC_FILE):
+    """HBM bytes per launch of a kernel class from a committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (256^3 workload only): dispatch-weighted
     mean over the class's kernels, or None when the summary is absent."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             lines = [ln.split() for ln in f if ln.strip().startswith("traffic ")]
     except OSError:
         return None
     n = tot = 0.0
     for _, name, cnt, fb, wb in lines:
-        if name.startswith(CLASS_KERNELS[cls]) and not name.startswith("col_inverse_flux<256,false>"):
+        # col_inverse_flux without the sponge is the max_speed diagnostic's pass
+        if name.startswith(CLASS_KERNELS[cls]) and not name.startswith("col_inverse_flux<256,false"):
             n += float(cnt)
             tot += float(cnt) * (float(fb) + float(wb))
     return tot / n if n else None
@@ -103,7 +106,8 @@ def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
             fp64_equivalent_tflops=ach, work_per_launch=work / n,
             work=f"4*M*K flops per complex band mode, executed as {npair} int8 slice products "
                  "(achieved counts int8 ops on the unpadded M x K; tile padding not counted)",
-            traffic=None)
+            traffic=profiled_traffic("gemm_i8_mixed", MIXED_TRAFFIC_FILE)
+            if full_size else None)
     ms, n, work = prof["shell_solve"]
     if n == 0:
         return out   # shell solves ran as contractions in the eigenbasis of B0
@@ -180,7 +184,8 @@ def mixed_leg(cfg, st0, args, ref, dref) -> dict:
     err.update({k: abs(d[k] - dref[k]) / abs(dref[k])
                 for k in ("KE_tot", "Nusselt_dealiased", "max_speed")})
     prof = sp.ctx.profile_step()
-    rl = rooflines(prof, g, sp.ctx.layout, full_size=False, slices=args.mixed_slices)
+    full = (args.nx, args.nz, args.dt) == (256, 256, 2.5e-5)
+    rl = rooflines(prof, g, sp.ctx.layout, full_size=full, slices=args.mixed_slices)
     out = {"value": args.steps / wall, "unit": "steps/s", "ms_per_step": 1e3 * wall / args.steps,
            "slices": args.mixed_slices,
            "dtype": f"f64 state; contractions as int8 x{args.mixed_slices} slice products "
