@@ -241,7 +241,9 @@ const char *nhqg_mlp_last_error(const nhqg_mlp *m);
  * explicit RHS of the step: E_q[n0 + j] += out_scale * MLP(in_scale *
  * q[n0 .. n0+15])_j for each mode and each of Re, Im.  The network must be
  * 16 -> 128 -> 128 -> 16 and created with NHQG_MLP_ODD (NHQG_ERR_ARG
- * otherwise); mlp = NULL detaches. */
+ * otherwise); mlp = NULL detaches.  The context keeps a BF16 snapshot of
+ * the weights taken by this call: training the network afterwards changes
+ * the step only after it is attached again. */
 int nhqg_set_closure(nhqg_ctx *ctx, const nhqg_mlp *mlp, int32_t n0, double in_scale,
                      double out_scale);
 

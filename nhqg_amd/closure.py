@@ -169,6 +169,24 @@ class Closure:
     def attach(self, grid: Grid) -> None:
         grid.context(full=False).set_closure(self.mlp, self.n0, self.in_scale, self.out_scale)
 
+    def save(self, path: str, **meta) -> None:
+        """FP32 parameters, scales and the top-level offset as .npz (loaded
+        without pickle)."""
+        np.savez(path, params=self.mlp.get_params(), n0=self.n0, in_scale=self.in_scale,
+                 out_scale=self.out_scale, loss=np.asarray(self.loss, np.float32),
+                 **{k: np.asarray(v) for k, v in meta.items()})
+
+    @staticmethod
+    def load(path: str, nz: int | None = None) -> "Closure":
+        """A saved closure on a new device MLP; nz places it on the top 16
+        levels of another Nz (default: the saved offset)."""
+        with np.load(path) as d:
+            mlp = _native.MLP(N_TOP, WIDTH, N_TOP, max_batch=4096, odd=True)
+            mlp.set_params(d["params"])
+            n0 = int(d["n0"]) if nz is None else nz + 1 - N_TOP
+            return Closure(mlp, n0, float(d["in_scale"]), float(d["out_scale"]),
+                           np.array(d["loss"]))
+
 
 def train(pairs: PairSet, steps: int = 3000, batch: int = 512, lr: float = 1e-3,
           seed: int = 0) -> Closure:
@@ -323,6 +341,7 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed-train", type=int, default=0)
     ap.add_argument("--seed-judge", type=int, default=10)
+    ap.add_argument("--save", type=str, default=None, help="write the trained closure (.npz)")
     ap.add_argument("--horizon", type=int, default=400,
                     help="1: one-step pairs; n > 1: n-step relaxation pairs from coarse runs")
     args = ap.parse_args(argv)
@@ -348,6 +367,9 @@ def main(argv=None) -> int:
     t_train = time.perf_counter() - t0
     print(f"trained in {t_train:.1f} s, loss {c.loss[:20].mean():.3e} -> {c.loss[-100:].mean():.3e}",
           flush=True)
+    if args.save:
+        c.save(args.save, nx=args.nx, nz_fine=args.nz_fine, nz_coarse=args.nz_coarse,
+               horizon=args.horizon, pairs=pairs.x.shape[0])
     gj = make_grid(cfg_f, shells="band")
     s_judge = seeded(gj, args.seed_judge)
     gj.close()

@@ -119,3 +119,67 @@ def polar_state_256(grid: Grid, seed: int = 0) -> State:
     barotropic flow of rms velocity 15."""
     s = thermal_noise_state(grid, seed=seed, amplitude=1e-2)
     return add_barotropic_flow(s, grid, seed=seed + 1, u_rms=15.0)
+
+
+def upsample_spectral(f_hat: np.ndarray, nx_new: int) -> np.ndarray:
+    """Horizontal spectral (sinc) upsampling of a stack of rfft2 spectra
+    (levels, Nx, Nx//2+1) to (levels, nx_new, nx_new//2+1), the method of
+    scripts/upsample_checkpoint_horiz.py:27-59: back to the old grid, full
+    FFT, the old Nyquist row and column split in half between +N/2 and
+    -N/2, zero padding around the centred spectrum, and the factor
+    (nx_new/Nx)^2 of the unnormalised forward FFT.  Grid values at the
+    co-located points are unchanged."""
+    f_hat = np.asarray(f_hat)
+    nl, nx, nk = f_hat.shape
+    if nk != nx // 2 + 1 or nx_new < nx or (nx_new - nx) % 2:
+        raise ValueError("upsample_spectral: need an rfft2 stack and nx_new >= Nx, same parity")
+    full = np.fft.fftshift(np.fft.fft2(np.fft.irfft2(f_hat, s=(nx, nx), axes=(1, 2)),
+                                       axes=(1, 2)), axes=(1, 2))
+    ext = np.zeros((nl, nx + 1, nx + 1), dtype=np.complex128)
+    ext[:, :nx, :nx] = full
+    # centred index 0 holds the -N/2 (Nyquist) row and column
+    ext[:, 0, :nx] *= 0.5
+    ext[:, nx, :nx] = ext[:, 0, :nx]
+    ext[:, :, 0] *= 0.5
+    ext[:, :, nx] = ext[:, :, 0]
+    pad = (nx_new - nx) // 2
+    big = np.zeros((nl, nx_new, nx_new), dtype=np.complex128)
+    big[:, pad:pad + nx + 1, pad:pad + nx + 1] = ext
+    out = np.fft.ifftshift(big, axes=(1, 2)) * (nx_new / nx) ** 2
+    return np.ascontiguousarray(out[:, :, : nx_new // 2 + 1])
+
+
+def upsample_state(state: State, grid_new: Grid) -> State:
+    """A state on an Nx grid carried to grid_new (same Nz, larger Nx) by
+    spectral upsampling of q', w, theta (th_bar is horizontal-mean only),
+    then sanitised for the new grid's 2/3 band."""
+    nx = grid_new.Nx
+    q, w, th = (upsample_spectral(f, nx) for f in (state.q_hat, state.w_hat, state.th_hat))
+    return sanitize_state(State(q, w, th, np.array(state.th_bar, dtype=np.float64)), grid_new)
+
+
+def inject_cyclones(state: State, grid: Grid, amplitude: float, radius: float, count: int = 1,
+                    rng: np.random.Generator | None = None) -> State:
+    """Gaussian cyclones added to the barotropic PV (the T_0 coefficient of
+    q') at random positions, 2/3-masked (scripts/run_polar.py:91-112)."""
+    rng = np.random.default_rng() if rng is None else rng
+    Nx, L = grid.Nx, grid.L
+    x = np.arange(Nx) * L / Nx
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    patch = np.zeros((Nx, Nx))
+    for _ in range(count):
+        x0, y0 = rng.uniform(0.0, L, size=2)
+        dx = (X - x0 + L / 2.0) % L - L / 2.0
+        dy = (Y - y0 + L / 2.0) % L - L / 2.0
+        patch += amplitude * np.exp(-(dx ** 2 + dy ** 2) / (2.0 * radius ** 2))
+    q = np.array(state.q_hat, dtype=np.complex128)
+    q[0] += np.fft.rfft2(patch) * grid.mask_23
+    return State(q, state.w_hat, state.th_hat, state.th_bar)
+
+
+def turbulent_state(grid: Grid, seed: int = 0, u_rms: float = 20.0,
+                    theta_rms: float = 0.1) -> State:
+    """Turbulent-amplitude seeded state: thermal noise of rms theta_rms plus
+    a barotropic flow of rms velocity u_rms."""
+    s = thermal_noise_state(grid, seed=seed, amplitude=theta_rms)
+    return add_barotropic_flow(s, grid, seed=seed + 1, u_rms=u_rms)
