@@ -43,10 +43,26 @@ NHQG_DEV double2 twid(const double2* tw, int e) {
   return INV ? make_double2(w.x, -w.y) : w;
 }
 
+// Barrier between the stages of one team's FFT.  A team of T <= 64 threads
+// lies inside one wavefront, whose LDS operations complete in issue order:
+// a wave-scope fence (a compiler barrier for the memory operations) is
+// enough, and the other teams of the block run on unhindered.  Larger
+// teams span several waves and take the block barrier.
+template <int T>
+NHQG_DEV void team_sync() {
+  if constexpr (T <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+}
+
 // Unnormalised DFT of N points held in LDS buffer a (b is scratch), by a
 // team of N/4 threads with index t.  INV selects exp(+i...).  Every thread
-// of the block must call it (block barriers inside).  Returns the buffer
-// that holds the result in natural order.
+// of the block must call it (team barriers inside; a caller that reads
+// another team's result after it needs a block barrier).  Returns the
+// buffer that holds the result in natural order.
 template <int N, bool INV>
 NHQG_DEV double2* team_fft(double2* a, double2* b, int t, const double2* tw) {
   constexpr int T = N / 4;
@@ -73,7 +89,7 @@ NHQG_DEV double2* team_fft(double2* a, double2* b, int t, const double2* tw) {
     out[d + Ns] = INV ? cadd(a1, ia3) : csub(a1, ia3);
     out[d + 2 * Ns] = csub(a0, a2);
     out[d + 3 * Ns] = INV ? csub(a1, ia3) : cadd(a1, ia3);
-    __syncthreads();
+    team_sync<T>();
     double2* tmp = in;
     in = out;
     out = tmp;
@@ -90,7 +106,7 @@ NHQG_DEV double2* team_fft(double2* a, double2* b, int t, const double2* tw) {
       out[d] = cadd(v0, v1);
       out[d + Ns] = csub(v0, v1);
     }
-    __syncthreads();
+    team_sync<T>();
     double2* tmp = in;
     in = out;
     out = tmp;
@@ -157,6 +173,7 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
     __syncthreads();
     double2* res = team_fft<N, true>(buf[0][team], buf[1][team], t, tw);
     const int par = (res == buf[0][team]) ? 0 : 1;
+    __syncthreads();   // the store below reads every team's result
     double2* dst = X5 + f * fstride + (size_t)l * N * nky;
     for (int idx = tid; idx < N * TEAMS; idx += 256) {
       const int cc = idx % TEAMS, x = idx / TEAMS;
@@ -197,7 +214,7 @@ NHQG_DEV void fwd_pair(double2* ba, double2* bb, int t, const double2* tw, const
   constexpr int T = N / 4;
 #pragma unroll
   for (int j = 0; j < 4; ++j) ba[t + j * T] = make_double2(a[j], b ? b[j] : 0.0);
-  __syncthreads();
+  team_sync<T>();
   const double2* r = team_fft<N, false>(ba, bb, t, tw);
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
@@ -212,7 +229,7 @@ NHQG_DEV void fwd_pair(double2* ba, double2* bb, int t, const double2* tw, const
       B[s] = make_double2(0.0, 0.0);
     }
   }
-  __syncthreads();
+  team_sync<T>();
 }
 
 template <int N>
@@ -220,7 +237,7 @@ NHQG_DEV void inv_pair(double2* ba, double2* bb, int t, const double2* tw, int n
                        const double2* P, const double2* Q, double* p, double* q, double scale) {
   constexpr int T = N / 4;
   build_packed<N>(ba, t, nky, P, Q);
-  __syncthreads();
+  team_sync<T>();
   const double2* r = team_fft<N, true>(ba, bb, t, tw);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -228,7 +245,7 @@ NHQG_DEV void inv_pair(double2* ba, double2* bb, int t, const double2* tw, int n
     p[j] = z.x * scale;
     if (q) q[j] = z.y * scale;
   }
-  __syncthreads();
+  team_sync<T>();
 }
 
 // Half spectra P, Q (entries ky < nky) at the packed slots k = t + j*N/4 a
@@ -267,7 +284,7 @@ NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, 
     else v = make_double2(a.x + b.y, b.x - a.y);
     ba[k] = v;
   }
-  __syncthreads();
+  team_sync<T>();
   const double2* r = team_fft<N, true>(ba, bb, t, tw);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -275,7 +292,7 @@ NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, 
     p[j] = z.x * scale;
     if (q) q[j] = z.y * scale;
   }
-  __syncthreads();
+  team_sync<T>();
 }
 
 // ---------------------------------------------------------------------------

@@ -81,10 +81,10 @@ __global__ void __launch_bounds__(GI_NW * 64)
 gemm_i8_kernel(GemmBatch bt) {
   constexpr int NT = GI_NW * 64;
   __shared__ v4i As[2][S * 4 * 64];   // [buf][(p * nk + kk) * 64 + lane]
+  __shared__ double sa_s[17 * 16], af_s[17 * 16];   // row scales and A[r][0] (nti <= 17)
 
   const GemmJob& J = bt.job[blockIdx.y];
   const v4i* __restrict__ A8 = (const v4i*)J.op.A8;
-  const double* __restrict__ sa = J.op.sa;
   const int M = J.op.M, K = J.op.K, nk = J.op.nk, nti = J.op.nti;
   const int KI = K - 1;   // sliced k = 1 .. K-1 (K <= 257)
   const int ld = bt.ld;
@@ -113,6 +113,10 @@ gemm_i8_kernel(GemmBatch bt) {
     }
   };
   fetchA(0);
+  for (int i = tid; i < nti * 16; i += NT) {
+    sa_s[i] = J.op.sa[i];
+    af_s[i] = J.op.afirst[i];
+  }
 
   // this lane's 64 values of its column (sliced index kl -> k = kl + 1),
   // all loads in flight at once; rows beyond K-1 are clamped and zeroed
@@ -195,7 +199,7 @@ gemm_i8_kernel(GemmBatch bt) {
       double s = (double)acc[S - 1][r];
 #pragma unroll
       for (int l = S - 2; l >= 0; --l) s = (double)acc[l][r] + s * 0.0078125;
-      v[r] = fma(J.op.afirst[row0 + r], bfirst, s * scale_c * sa[row0 + r]);
+      v[r] = fma(af_s[row0 + r], bfirst, s * scale_c * sa_s[row0 + r]);
     }
     if (J.C2) epi_rows4(J.e2, J.C2, v, row0, M, ld, col, mode);
     epi_rows4(J.e, J.C, v, row0, M, ld, col, mode);
@@ -212,7 +216,8 @@ hipError_t launch_gemm_i8(const GemmBatch& b, int nj, hipStream_t s) {
   for (int j = 0; j < nj; ++j) {
     const GemmJob& J = b.job[j];
     const GemmOp& op = J.op;
-    if (!op.A8 || !op.sa || !op.afirst || op.slices != b.slices || op.M < 1 || op.K < 2 ||
+    if (!op.A8 || !op.sa || !op.afirst || op.slices != b.slices || op.M < 1 || op.M > 257 ||
+        op.K < 2 ||
         op.K > 257 || op.nk != (op.K - 1 + 63) / 64 ||
         op.nti != (op.M + 15) / 16 || J.W || !J.B || !J.C)
       return hipErrorInvalidValue;
