@@ -63,6 +63,7 @@ struct nhqg_ctx {
   double *wsbp, *thsbp, *thsbp2;
   double *spq, *spw, *spth;
   double *partial, *sbp, *g;
+  double *sbp_xy, *sbp_map, *thbt;   // corrector substep map (X|Y, [M1 M2; G1 G2]) and scratch
   double2 *X5, *Y6;
   size_t fstride;
   double2* staging;
@@ -432,15 +433,20 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
                      c->st));
   PL(CL_SBP, 0.0, launch_sbp_prepare(n, c->Pred, c->partial, o.d1b, o.lb, 0.5 * mu * kappa * sdt,
                         0.25 * mu * sdt * sdt, c->sbp, c->st));
-  if (thb_out != thb_in)
-    PL(CL_SMALL, 0.0, launch_small_axpy(n, thb_in, thb_in, thb_in, 0.0, thb_out, c->st));
+  PL(CL_SBP, 0.0, launch_sbp_maps(n, c->sbp, o.thb2s, o.s2thb, o.d1b, mu * sdt, c->sbp_xy,
+                                  c->sbp_map, c->st));
+  // th_bar ping-pongs between thb_out and a scratch level vector; an even
+  // number of substeps ends in thb_out (an odd one gets a final copy)
+  double* tb_a = c->thbt;
+  double* tb_b = thb_out;
+  if (ns % 2 == 0) std::swap(tb_a, tb_b);
+  PL(CL_SMALL, 0.0, launch_small_axpy(n, thb_in, thb_in, thb_in, 0.0, tb_a, c->st));
   for (int s = 0; s < ns; ++s) {
     PL(CL_PARSEVAL, 32.0 * n * (double)c->nm,
        launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, cur, o.pw, o.partner, c->partial,
                        c->st));
-    PL(CL_SBP, 0.0,
-       launch_sbp_substep(n, c->Pred, c->partial, o.d1b, o.thb2s, o.s2thb, mu * sdt, c->sbp,
-                          thb_out, c->g, c->st));
+    PL(CL_SBP, 0.0, launch_sbp_apply(n, c->Pred, c->partial, c->sbp_map, tb_a, tb_b, c->g, c->st));
+    std::swap(tb_a, tb_b);
     Job j{s + 1 < ns ? &o.Psbp : &o.S2T, cur, s + 1 < ns ? nxt : th_out, epi_make(1.0)};
     j.W = c->wsbp;
     j.g = c->g;
@@ -448,6 +454,9 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
     RUN(gemm_jobs(c, {j}));
     std::swap(cur, nxt);
   }
+  // after the loop tb_a holds the last result
+  if (tb_a != thb_out)
+    PL(CL_SMALL, 0.0, launch_small_axpy(n, tb_a, tb_a, tb_a, 0.0, thb_out, c->st));
   return NHQG_OK;
 }
 
@@ -835,6 +844,9 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(field(c, &c->spth, Ng));
   RUN(dalloc(c, &c->partial, (size_t)Nq * c->Pred + 64));
   RUN(dalloc(c, &c->sbp, (size_t)Nq * 16));
+  RUN(dalloc(c, &c->sbp_xy, (size_t)Nq * 2 * Nq));
+  RUN(dalloc(c, &c->sbp_map, (size_t)4 * Nq * Nq));
+  RUN(dalloc(c, &c->thbt, (size_t)Nq));
   RUN(dalloc(c, &c->g, (size_t)Nq));
   // FFT intermediates, one block of 17 fields: flux form 8 in + 9 out,
   // Jacobian form 11 in + 6 out

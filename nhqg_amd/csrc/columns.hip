@@ -474,109 +474,220 @@ sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const doubl
   for (int i = tid; i < n; i += 512) LUg[5 * n + i] = rd[i];
 }
 
-// One substep's 1-D solve: flux -> rhs -> thb_sbp_new = A^{-1} rhs (banded
-// forward and back substitution with the preamble's LU, one thread) -> g, thb.
-__global__ void __launch_bounds__(1024)
-sbp_substep_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ d1b,
-                   const double* __restrict__ thb2s, const double* __restrict__ s2thb,
-                   double mu_sdt, const double* __restrict__ Bg, const double* __restrict__ LUg,
-                   double* __restrict__ thb_io, double* __restrict__ g_out) {
-  __shared__ double fl[SBP_NMAX], tb[SBP_NMAX], rh[SBP_NMAX], nw[SBP_NMAX], th0[SBP_NMAX];
-  __shared__ double LU[6 * SBP_NMAX], Bs[5 * SBP_NMAX], d1s[3 * SBP_NMAX];
+// ---------------------------------------------------------------------------
+// SBP substeps as one affine map per corrector call.  Within a call the
+// pentadiagonal A (and B, D1, mu dt_s) are fixed, and a substep
+// (nhqg/solver.py:885-905)
+//   tb = thb2s th, rh = Zr (B tb - mu dt_s D1 F), nw = A^-1 rh,
+//   g = D1 (tb + nw) / 2, th' = s2thb Z nw
+// (Zr, Z zero the two boundary rows) is affine in (th, F):
+//   th' = M1 th + M2 F,  g = G1 th + G2 F
+// with X = A^-1 Zr B thb2s, Y = A^-1 Zr D1 and
+//   M1 = s2thb Z X, M2 = -mu dt_s s2thb Z Y,
+//   G1 = (D1 thb2s + D1 X) / 2, G2 = -mu dt_s D1 Y / 2.
+// sbp_cols_kernel solves the 2n banded systems for X and Y (one thread per
+// column, the LU recurrences of the preamble), sbp_compose_kernel forms the
+// 2n x 2n map [M1 M2; G1 G2] row-major, and each substep is one many-block
+// matvec (sbp_apply_kernel) instead of a single-workgroup chain.
+// ---------------------------------------------------------------------------
+// Right-hand sides of the column solve, one element per thread, into X|Y
+// (n x 2n row-major): Zr (B thb2s) in columns [0, n), Zr D1 in [n, 2n).
+__global__ void __launch_bounds__(256)
+sbp_rhs_kernel(int n, const double* __restrict__ Bg, const double* __restrict__ thb2s,
+               const double* __restrict__ d1b, double* __restrict__ XY) {
+  const int ld = 2 * n;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n * ld) return;
+  const int i = idx / ld, col = idx % ld;
+  double r = 0.0;
+  if (i > 0 && i < n - 1) {
+    if (col < n) {
+#pragma unroll
+      for (int o = -2; o <= 2; ++o) {
+        const int j = i + o;
+        if (j >= 0 && j < n) r += Bg[i * 5 + o + 2] * thb2s[(size_t)j * n + col];
+      }
+    } else {
+      r = band3(d1b, i, col - n, n);
+    }
+  }
+  XY[idx] = r;
+}
+
+// In-place banded solve of every column of X|Y with the preamble's LU, one
+// thread per column.  The column streams through registers in chunks of
+// SBP_CH rows, the next chunk (values and LU coefficients) loaded before
+// the current one runs through the dependent recurrence.
+#define SBP_CH 8
+__global__ void __launch_bounds__(64)
+sbp_cols_kernel(int n, const double* __restrict__ LUg, double* __restrict__ XY) {
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  const int ld = 2 * n;
+  if (col >= ld) return;
+  const double* rdp = LUg + 5 * n;
+  double* cp = XY + col;
+  const int nch = (n + SBP_CH - 1) / SBP_CH;
+  // forward: y_i = r_i - (b_i y_{i-1} + a_i y_{i-2})
+  double rv[SBP_CH], av[SBP_CH], bv[SBP_CH], rn[SBP_CH], an[SBP_CH], bn[SBP_CH];
+  auto fload = [&](int c, double (&r)[SBP_CH], double (&a)[SBP_CH], double (&b)[SBP_CH]) {
+#pragma unroll
+    for (int u = 0; u < SBP_CH; ++u) {
+      const int i = min(c * SBP_CH + u, n - 1);
+      r[u] = cp[(size_t)i * ld];
+      a[u] = LUg[i * 5 + 0];
+      b[u] = LUg[i * 5 + 1];
+    }
+  };
+  fload(0, rn, an, bn);
+  double y1 = 0.0, y2 = 0.0;
+  for (int c = 0; c < nch; ++c) {
+#pragma unroll
+    for (int u = 0; u < SBP_CH; ++u) {
+      rv[u] = rn[u];
+      av[u] = an[u];
+      bv[u] = bn[u];
+    }
+    if (c + 1 < nch) fload(c + 1, rn, an, bn);
+#pragma unroll
+    for (int u = 0; u < SBP_CH; ++u) {
+      const int i = c * SBP_CH + u;
+      if (i < n) {
+        const double v = rv[u] - (bv[u] * y1 + av[u] * y2);
+        cp[(size_t)i * ld] = v;
+        y2 = y1;
+        y1 = v;
+      }
+    }
+  }
+  // backward: x_i = (y_i - e_i x_{i+1} - f_i x_{i+2}) / d_i
+  double ev[SBP_CH], fv[SBP_CH], dv[SBP_CH], en[SBP_CH], fn[SBP_CH], dn[SBP_CH];
+  auto bload = [&](int c) {
+#pragma unroll
+    for (int u = 0; u < SBP_CH; ++u) {
+      const int i = max(n - 1 - (c * SBP_CH + u), 0);
+      rn[u] = cp[(size_t)i * ld];
+      en[u] = LUg[i * 5 + 3];
+      fn[u] = LUg[i * 5 + 4];
+      dn[u] = rdp[i];
+    }
+  };
+  bload(0);
+  double x1 = 0.0, x2 = 0.0;
+  for (int c = 0; c < nch; ++c) {
+#pragma unroll
+    for (int u = 0; u < SBP_CH; ++u) {
+      rv[u] = rn[u];
+      ev[u] = en[u];
+      fv[u] = fn[u];
+      dv[u] = dn[u];
+    }
+    if (c + 1 < nch) bload(c + 1);
+#pragma unroll
+    for (int u = 0; u < SBP_CH; ++u) {
+      const int i = n - 1 - (c * SBP_CH + u);
+      if (i >= 0) {
+        const double v = (rv[u] - ev[u] * x1 - fv[u] * x2) * dv[u];
+        cp[(size_t)i * ld] = v;
+        x2 = x1;
+        x1 = v;
+      }
+    }
+  }
+}
+
+// Mall (2n x 2n, row-major) = [M1 M2; G1 G2].  Rows [0, n): s2thb Z XY as a
+// 32 x 32-tiled product through LDS (4 outputs per thread), the Y half
+// scaled by -mu dt_s; rows [n, 2n): the banded D1 rows, one element per
+// thread.
+#define SBP_TILE 32
+__global__ void __launch_bounds__(256)
+sbp_compose_kernel(int n, const double* __restrict__ XY, const double* __restrict__ s2thb,
+                   const double* __restrict__ thb2s, const double* __restrict__ d1b, double mu_sdt,
+                   double* __restrict__ Mall) {
+  const int ld = 2 * n;
   const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += blockDim.x) th0[i] = thb_io[i];
-  for (int i = tid; i < 6 * n; i += blockDim.x) LU[i] = LUg[i];
-  for (int i = tid; i < 5 * n; i += blockDim.x) Bs[i] = Bg[i];
-  for (int i = tid; i < 3 * n; i += blockDim.x) d1s[i] = d1b[i];
-  for (int i = tid; i < n; i += blockDim.x) {
+  const int ntr = (n + SBP_TILE - 1) / SBP_TILE, ntc = (ld + SBP_TILE - 1) / SBP_TILE;
+  if ((int)blockIdx.x < ntr * ntc) {
+    __shared__ double As[SBP_TILE][SBP_TILE + 1], Bs[SBP_TILE][SBP_TILE + 1];
+    const int r0 = (blockIdx.x / ntc) * SBP_TILE, c0 = (blockIdx.x % ntc) * SBP_TILE;
+    const int tc = tid % SBP_TILE, tr = tid / SBP_TILE;   // 8 row groups of 4 rows
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < n; k0 += SBP_TILE) {
+      for (int e = tid; e < SBP_TILE * SBP_TILE; e += 256) {
+        const int a = e / SBP_TILE, b = e % SBP_TILE;
+        const int r = r0 + a, k = k0 + b;
+        As[a][b] = (r < n && k < n) ? s2thb[(size_t)r * n + k] : 0.0;
+        const int kb = k0 + a, cb = c0 + b;
+        // Z: rows 0 and n-1 of XY do not contribute
+        Bs[a][b] = (kb > 0 && kb < n - 1 && cb < ld) ? XY[(size_t)kb * ld + cb] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (int kk = 0; kk < SBP_TILE; ++kk) {
+        const double bv = Bs[kk][tc];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = fma(As[tr * 4 + q][kk], bv, acc[q]);
+      }
+      __syncthreads();
+    }
+    const int c = c0 + tc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + tr * 4 + q;
+      if (r < n && c < ld) Mall[(size_t)r * ld + c] = (c >= n) ? -mu_sdt * acc[q] : acc[q];
+    }
+    return;
+  }
+  const int idx = (blockIdx.x - ntr * ntc) * 256 + tid;
+  if (idx >= n * ld) return;
+  const int i = idx / ld, c = idx % ld;
+  double v = 0.0;
+#pragma unroll
+  for (int o = -1; o <= 1; ++o) {
+    const int j = i + o;
+    if (j < 0 || j >= n) continue;
+    const double dij = band3(d1b, i, j, n);
+    v += dij * (c < n ? thb2s[(size_t)j * n + c] + XY[(size_t)j * ld + c] : XY[(size_t)j * ld + c]);
+  }
+  Mall[(size_t)(n + i) * ld + c] = v * ((c < n) ? 0.5 : -0.5 * mu_sdt);
+}
+
+// One substep: F = sum_p partial[:, p] (each block redoes this small sum),
+// x = [th_src; F], out = Mall x; rows [0, n) -> th_dst, rows [n, 2n) -> g.
+#define SBP_APPLY_ROWS 8
+__global__ void __launch_bounds__(256)
+sbp_apply_kernel(int n, int P, const double* __restrict__ partial, const double* __restrict__ Mall,
+                 const double* __restrict__ th_src, double* __restrict__ th_dst,
+                 double* __restrict__ g_out) {
+  __shared__ double x[2 * SBP_NMAX];
+  const int tid = threadIdx.x, ld = 2 * n;
+  for (int i = tid; i < n; i += 256) {
+    x[i] = th_src[i];
     double sum = 0.0;
     for (int p = 0; p < P; ++p) sum += partial[(size_t)i * P + p];
-    fl[i] = sum;
+    x[n + i] = sum;
   }
   __syncthreads();
-  block_matvec(n, n, thb2s, th0, tb);
-  __syncthreads();
-  for (int i = tid; i < n; i += blockDim.x) {
-    double r = 0.0;
-    for (int o = -2; o <= 2; ++o) {
-      const int j = i + o;
-      if (j >= 0 && j < n) r += Bs[i * 5 + o + 2] * tb[j];
-    }
-    double d = 0.0;
-    for (int o = -1; o <= 1; ++o) {
-      const int j = i + o;
-      if (j >= 0 && j < n) d += band3(d1s, i, j, n) * fl[j];
-    }
-    r -= mu_sdt * d;
-    rh[i] = (i == 0 || i == n - 1) ? 0.0 : r;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    // L y = rh (unit lower, multipliers at the a, b slots), U x = y; in
-    // chunks of 8 rows whose LDS operands are loaded before the dependent
-    // chain runs through them
-    constexpr int C = 8;
-    const double* rdp = LU + 5 * n;
-    double y1 = 0.0, y2 = 0.0;
-    for (int i0 = 0; i0 < n; i0 += C) {
-      double r[C], l1[C], l0[C];
+  const int wave = tid >> 6, lane = tid & 63;
+  constexpr int RW = SBP_APPLY_ROWS / 4;   // rows per wave
+  const int r0 = blockIdx.x * SBP_APPLY_ROWS + wave * RW;
+  double t[RW];
 #pragma unroll
-      for (int u = 0; u < C; ++u) {
-        const int i = min(i0 + u, n - 1);
-        r[u] = rh[i];
-        l1[u] = LU[i * 5 + 1];
-        l0[u] = LU[i * 5 + 0];
-      }
+  for (int q = 0; q < RW; ++q) t[q] = 0.0;
+  for (int c = lane; c < ld; c += 64) {
+    const double xc = x[c];
 #pragma unroll
-      for (int u = 0; u < C; ++u) {
-        if (i0 + u < n) {
-          const double v = r[u] - (l1[u] * y1 + l0[u] * y2);
-          nw[i0 + u] = v;
-          y2 = y1;
-          y1 = v;
-        }
-      }
-    }
-    // x_i = (y_i - e_i x_{i+1} - f_i x_{i+2}) / d_i; e, f of the last rows
-    // are zero (band entries outside the matrix)
-    double x1 = 0.0, x2 = 0.0;
-    for (int i0 = n - 1; i0 >= 0; i0 -= C) {
-      double yv[C], ev[C], fv[C], rv[C];
+    for (int q = 0; q < RW; ++q) t[q] = fma(Mall[(size_t)min(r0 + q, ld - 1) * ld + c], xc, t[q]);
+  }
 #pragma unroll
-      for (int u = 0; u < C; ++u) {
-        const int i = max(i0 - u, 0);
-        yv[u] = nw[i];
-        ev[u] = LU[i * 5 + 3];
-        fv[u] = LU[i * 5 + 4];
-        rv[u] = rdp[i];
-      }
-#pragma unroll
-      for (int u = 0; u < C; ++u) {
-        if (i0 - u >= 0) {
-          const double v = (yv[u] - ev[u] * x1 - fv[u] * x2) * rv[u];
-          nw[i0 - u] = v;
-          x2 = x1;
-          x1 = v;
-        }
-      }
+  for (int q = 0; q < RW; ++q) {
+    const double v = wave_sum(t[q]);
+    const int r = r0 + q;
+    if (lane == 0 && r < ld) {
+      if (r < n) th_dst[r] = v;
+      else g_out[r - n] = v;
     }
   }
-  __syncthreads();
-  for (int i = tid; i < n; i += blockDim.x) {
-    double gs = 0.0;
-    for (int o = -1; o <= 1; ++o) {
-      const int j = i + o;
-      if (j >= 0 && j < n) gs += band3(d1s, i, j, n) * (tb[j] + nw[j]);
-    }
-    g_out[i] = 0.5 * gs;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    nw[0] = 0.0;
-    nw[n - 1] = 0.0;
-  }
-  __syncthreads();
-  block_matvec(n, n, s2thb, nw, thb_io);
 }
 
 // sbp buffer (doubles, n = Nz+1): [0,5n) B band, [5n,10n) LU bands,
@@ -592,14 +703,27 @@ hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double*
   return hipGetLastError();
 }
 
-hipError_t launch_sbp_substep(int n, int P, const double* partial, const double* d1b,
-                              const double* thb2s, const double* s2thb, double mu_sdt,
-                              double* sbp, double* thb_io, double* g_out, hipStream_t s) {
+hipError_t launch_sbp_maps(int n, const double* sbp, const double* thb2s, const double* s2thb,
+                           const double* d1b, double mu_sdt, double* XY, double* Mall,
+                           hipStream_t s) {
   if (n > SBP_NMAX) return hipErrorInvalidValue;
   const double* Bg = sbp;
   const double* LUg = sbp + 5 * (size_t)n;
-  hipLaunchKernelGGL(sbp_substep_kernel, dim3(1), dim3(1024), 0, s, n, P, partial, d1b, thb2s,
-                     s2thb, mu_sdt, Bg, LUg, thb_io, g_out);
+  hipLaunchKernelGGL(sbp_rhs_kernel, dim3((2 * n * n + 255) / 256), dim3(256), 0, s, n, Bg, thb2s,
+                     d1b, XY);
+  hipLaunchKernelGGL(sbp_cols_kernel, dim3((2 * n + 63) / 64), dim3(64), 0, s, n, LUg, XY);
+  const int ntr = (n + SBP_TILE - 1) / SBP_TILE, ntc = (2 * n + SBP_TILE - 1) / SBP_TILE;
+  const int nband = (2 * n * n + 255) / 256;
+  hipLaunchKernelGGL(sbp_compose_kernel, dim3(ntr * ntc + nband), dim3(256), 0, s, n, XY, s2thb,
+                     thb2s, d1b, mu_sdt, Mall);
+  return hipGetLastError();
+}
+
+hipError_t launch_sbp_apply(int n, int P, const double* partial, const double* Mall,
+                            const double* th_src, double* th_dst, double* g_out, hipStream_t s) {
+  if (n > SBP_NMAX || th_src == th_dst) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(sbp_apply_kernel, dim3((2 * n + SBP_APPLY_ROWS - 1) / SBP_APPLY_ROWS),
+                     dim3(256), 0, s, n, P, partial, Mall, th_src, th_dst, g_out);
   return hipGetLastError();
 }
 

@@ -126,9 +126,13 @@ hipError_t launch_parseval(int nl, int nm, int ld, int P, const double* a, const
 hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double* d1b,
                               const double* lb, double c1, double c2, double* sbp,
                               hipStream_t s);
-hipError_t launch_sbp_substep(int n, int P, const double* partial, const double* d1b,
-                              const double* thb2s, const double* s2thb, double mu_sdt,
-                              double* sbp, double* thb_io, double* g_out, hipStream_t s);
+// X, Y and the 2n x 2n substep map of the corrector call (columns.hip)
+hipError_t launch_sbp_maps(int n, const double* sbp, const double* thb2s, const double* s2thb,
+                           const double* d1b, double mu_sdt, double* XY, double* Mall,
+                           hipStream_t s);
+// one substep: th_dst, g from th_src and the Parseval partials
+hipError_t launch_sbp_apply(int n, int P, const double* partial, const double* Mall,
+                            const double* th_src, double* th_dst, double* g_out, hipStream_t s);
 hipError_t launch_small_matvec(int n_out, int n_in, const double* M, const double* x, double* y,
                                hipStream_t s);
 hipError_t launch_small_axpy(int n, const double* x0, const double* x1, const double* x2, double c,
