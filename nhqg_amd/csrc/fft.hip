@@ -19,8 +19,10 @@
 //   col_forward : per (level, ky): forward DFT along x, i kx G + H,
 //                 gather of the kept rows (2/3 band)                -> A
 //
-// A "team" of Nx/4 threads owns one transform; a 256-thread block holds
-// 1024/Nx teams.  The trap background is added to q' in physical space.
+// A "team" of Nx/4 threads owns one transform.  Row passes run 256-thread
+// blocks (1024/Nx teams); the column passes run ColBlock<N> threads, so a
+// block holds at least 4 columns and their transposed stores and gathers
+// move >= 64-byte runs (4 columns at Nx = 1024, 8 at 512).  The trap background is added to q' in physical space.
 // The sponge rides in the same passes but, as in the reference, acts on
 // the coefficient levels of q', w and theta (X5 slots 5-7, Y6 slots 6-8):
 // acting on nodal values instead is equal in exact arithmetic, but the
@@ -36,6 +38,11 @@ template <> struct Log2<128> { static constexpr int v = 7; };
 template <> struct Log2<256> { static constexpr int v = 8; };
 template <> struct Log2<512> { static constexpr int v = 9; };
 template <> struct Log2<1024> { static constexpr int v = 10; };
+
+template <int N>
+struct ColBlock {
+  static constexpr int v = (N >= 512) ? 1024 : 256;
+};
 
 template <bool INV>
 NHQG_DEV double2 twid(const double2* tw, int e) {
@@ -128,7 +135,7 @@ NHQG_DEV void load_twiddles(double2* tw, const double2* twg) {
 // then the sponge inputs.
 // ---------------------------------------------------------------------------
 template <int N, bool SPONGE, bool JAC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(ColBlock<N>::v)
 col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ qn,
                  const double2* __restrict__ wn, const double2* __restrict__ thn,
                  const double2* __restrict__ qc, const double2* __restrict__ wc,
@@ -136,7 +143,8 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
                  const double* __restrict__ kx_row, const double* __restrict__ ky_col,
                  const double* __restrict__ inv_denom, const double2* __restrict__ twg,
                  double2* __restrict__ X5, size_t fstride) {
-  constexpr int T = N / 4, TEAMS = 256 / T;
+  constexpr int BS = ColBlock<N>::v;
+  constexpr int T = N / 4, TEAMS = BS / T;
   constexpr int NA = JAC ? 8 : 5;   // advection inputs
   constexpr int NF = NA + (SPONGE ? 3 : 0);
   __shared__ double2 buf[2][TEAMS][N];
@@ -175,7 +183,7 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
     const int par = (res == buf[0][team]) ? 0 : 1;
     __syncthreads();   // the store below reads every team's result
     double2* dst = X5 + f * fstride + (size_t)l * N * nky;
-    for (int idx = tid; idx < N * TEAMS; idx += 256) {
+    for (int idx = tid; idx < N * TEAMS; idx += BS) {
       const int cc = idx % TEAMS, x = idx / TEAMS;
       const int kyc = ky0 + cc;
       if (kyc < nky) dst[(size_t)x * nky + kyc] = buf[par][cc][x];
@@ -497,14 +505,15 @@ row_jac(int nky, int ng, const double2* __restrict__ X, size_t fstride, double2*
 // A into the nodal buffers.
 // ---------------------------------------------------------------------------
 template <int N, bool SPONGE, bool JAC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(ColBlock<N>::v)
 col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ Y6,
                  size_t fstride, const int* __restrict__ rows, const double* __restrict__ kx_row,
                  const double* __restrict__ mask, double2* __restrict__ out_q,
                  double2* __restrict__ out_w, double2* __restrict__ out_th,
                  double2* __restrict__ sp_q, double2* __restrict__ sp_w,
                  double2* __restrict__ sp_th, const double2* __restrict__ twg) {
-  constexpr int T = N / 4, TEAMS = 256 / T;
+  constexpr int BS = ColBlock<N>::v;
+  constexpr int T = N / 4, TEAMS = BS / T;
   __shared__ double2 buf[2][TEAMS][N];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
@@ -518,7 +527,7 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
   constexpr int NA = JAC ? 1 : 2;     // G (and H) per field
   constexpr int NG = NA + (SPONGE ? 1 : 0);
   constexpr int NQ = 3 * NG;
-  constexpr int NL = N * TEAMS / 256; // slot values gathered per thread
+  constexpr int NL = N * TEAMS / BS;  // slot values gathered per thread
   // output rows, kx and mask of this thread's modes, the same for every field
   int rowv[4];
   double kxv[4], mkv[4];
@@ -538,7 +547,7 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
     const double2* src = Y6 + slot * fstride + (size_t)l * N * nky;
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
-      const int idx = tid + it * 256;
+      const int idx = tid + it * BS;
       const int cc = idx % TEAMS, x = idx / TEAMS;
       const int kyc = ky0 + cc;
       v[it] = (kyc < nky) ? src[(size_t)x * nky + kyc] : make_double2(0.0, 0.0);
@@ -557,7 +566,7 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
     if (qn < NQ) gather(qn, nxt);
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
-      const int idx = tid + it * 256;
+      const int idx = tid + it * BS;
       buf[0][idx % TEAMS][idx / TEAMS] = cur[it];
     }
     __syncthreads();
@@ -626,11 +635,12 @@ row_speed(int nky, const double2* __restrict__ X5, size_t fstride, double* __res
 
 template <int N, bool SP, bool JAC>
 static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
-  constexpr int TEAMS = 1024 / N;
-  dim3 gc((a.nky + TEAMS - 1) / TEAMS, a.nq);
+  constexpr int TEAMS = 1024 / N;                       // row passes
+  constexpr int BC = ColBlock<N>::v, TC = 4 * BC / N;   // column passes
+  dim3 gc((a.nky + TC - 1) / TC, a.nq);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
   if (pass == 0 || pass == 1)
-    hipLaunchKernelGGL((col_inverse_flux<N, SP, JAC>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc,
+    hipLaunchKernelGGL((col_inverse_flux<N, SP, JAC>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc,
                        a.ng, a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row, a.ky_col,
                        a.inv_denom, a.tw, a.X5, a.fstride);
   if ((pass == 0 || pass == 2) && JAC) {
@@ -649,7 +659,7 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
                          a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
   }
   if (pass == 0 || pass == 3)
-    hipLaunchKernelGGL((col_forward_flux<N, SP, JAC>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc,
+    hipLaunchKernelGGL((col_forward_flux<N, SP, JAC>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc,
                        a.ng, a.Y6, a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath, a.sq,
                        a.sw, a.sth, a.tw);
 }
@@ -682,8 +692,9 @@ hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s) {
 template <int N>
 static hipError_t speed_n(const FluxArgs& a, double* rowmax, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;
-  dim3 gc((a.nky + TEAMS - 1) / TEAMS, a.nq);
-  hipLaunchKernelGGL((col_inverse_flux<N, false, false>), gc, dim3(256), 0, s, a.nkx, a.nky, a.ldc, 0,
+  constexpr int BC = ColBlock<N>::v, TC = 4 * BC / N;
+  dim3 gc((a.nky + TC - 1) / TC, a.nq);
+  hipLaunchKernelGGL((col_inverse_flux<N, false, false>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc, 0,
                      a.qn, a.qn, a.qn, a.qn, a.qn, a.qn, a.kidx_of_x, a.kx_row, a.ky_col,
                      a.inv_denom, a.tw, a.X5, a.fstride);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
