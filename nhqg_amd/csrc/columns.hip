@@ -514,31 +514,35 @@ sbp_rhs_kernel(int n, const double* __restrict__ Bg, const double* __restrict__ 
   XY[idx] = r;
 }
 
-// In-place banded solve of every column of X|Y with the preamble's LU, one
-// thread per column.  The column streams through registers in chunks of
-// SBP_CH rows, the next chunk (values and LU coefficients) loaded before
-// the current one runs through the dependent recurrence.
+// Banded solve of every column of X|Y with the preamble's LU, one thread
+// per column (64 per block).  Inputs stream from global memory in chunks of
+// SBP_CH rows, the next chunk loaded before the current one runs through
+// the dependent recurrence; intermediate values live in LDS and the column
+// is written back once at the end, so no global store sits in front of a
+// load wait (stores and loads share the vmcnt counter).
 #define SBP_CH 8
 __global__ void __launch_bounds__(64)
 sbp_cols_kernel(int n, const double* __restrict__ LUg, double* __restrict__ XY) {
-  const int col = blockIdx.x * 64 + threadIdx.x;
+  __shared__ double cs[SBP_NMAX * 64];
+  const int t = threadIdx.x;
+  const int col = blockIdx.x * 64 + t;
   const int ld = 2 * n;
-  if (col >= ld) return;
+  const bool on = col < ld;
   const double* rdp = LUg + 5 * n;
-  double* cp = XY + col;
+  const double* cp = XY + min(col, ld - 1);
   const int nch = (n + SBP_CH - 1) / SBP_CH;
   // forward: y_i = r_i - (b_i y_{i-1} + a_i y_{i-2})
   double rv[SBP_CH], av[SBP_CH], bv[SBP_CH], rn[SBP_CH], an[SBP_CH], bn[SBP_CH];
-  auto fload = [&](int c, double (&r)[SBP_CH], double (&a)[SBP_CH], double (&b)[SBP_CH]) {
+  auto fload = [&](int c) {
 #pragma unroll
     for (int u = 0; u < SBP_CH; ++u) {
       const int i = min(c * SBP_CH + u, n - 1);
-      r[u] = cp[(size_t)i * ld];
-      a[u] = LUg[i * 5 + 0];
-      b[u] = LUg[i * 5 + 1];
+      rn[u] = cp[(size_t)i * ld];
+      an[u] = LUg[i * 5 + 0];
+      bn[u] = LUg[i * 5 + 1];
     }
   };
-  fload(0, rn, an, bn);
+  fload(0);
   double y1 = 0.0, y2 = 0.0;
   for (int c = 0; c < nch; ++c) {
 #pragma unroll
@@ -547,13 +551,13 @@ sbp_cols_kernel(int n, const double* __restrict__ LUg, double* __restrict__ XY) 
       av[u] = an[u];
       bv[u] = bn[u];
     }
-    if (c + 1 < nch) fload(c + 1, rn, an, bn);
+    if (c + 1 < nch) fload(c + 1);
 #pragma unroll
     for (int u = 0; u < SBP_CH; ++u) {
       const int i = c * SBP_CH + u;
       if (i < n) {
         const double v = rv[u] - (bv[u] * y1 + av[u] * y2);
-        cp[(size_t)i * ld] = v;
+        cs[i * 64 + t] = v;
         y2 = y1;
         y1 = v;
       }
@@ -565,7 +569,7 @@ sbp_cols_kernel(int n, const double* __restrict__ LUg, double* __restrict__ XY) 
 #pragma unroll
     for (int u = 0; u < SBP_CH; ++u) {
       const int i = max(n - 1 - (c * SBP_CH + u), 0);
-      rn[u] = cp[(size_t)i * ld];
+      rn[u] = cs[i * 64 + t];   // y from the forward pass (all rows written)
       en[u] = LUg[i * 5 + 3];
       fn[u] = LUg[i * 5 + 4];
       dn[u] = rdp[i];
@@ -587,12 +591,14 @@ sbp_cols_kernel(int n, const double* __restrict__ LUg, double* __restrict__ XY) 
       const int i = n - 1 - (c * SBP_CH + u);
       if (i >= 0) {
         const double v = (rv[u] - ev[u] * x1 - fv[u] * x2) * dv[u];
-        cp[(size_t)i * ld] = v;
+        cs[i * 64 + t] = v;
         x2 = x1;
         x1 = v;
       }
     }
   }
+  if (on)
+    for (int i = 0; i < n; ++i) XY[(size_t)i * ld + col] = cs[i * 64 + t];
 }
 
 // Mall (2n x 2n, row-major) = [M1 M2; G1 G2].  Rows [0, n): s2thb Z XY as a
