@@ -10,14 +10,25 @@ an anti-Hermitian ghost there).  Files are read with numpy's default
 ``run_loop`` follows the reference driver (scripts/run_polar.py:332-378):
 advance in chunks of ``diag_steps`` on the device, compute the scalar
 diagnostics after each chunk, append a CSV row, and on any non-finite value
-save a checkpoint and stop with status 2.
+save a checkpoint and stop with status 2.  Like the reference driver
+(scripts/run_polar.py:233-268) it first records the invocation and the
+resolved config in ``run_config.json`` (``run_config.<n>.json`` when an
+earlier leg's record exists); ``read_run_config`` rebuilds the config for a
+restart.
 """
 
 from __future__ import annotations
 
 import csv
+import dataclasses
+import datetime
+import json
 import math
 import os
+import platform
+import shlex
+import socket
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -62,6 +73,47 @@ def load_checkpoint(path: str | os.PathLike) -> tuple[State, int, float]:
         return State(q, w, th, thb), int(d["step"]), float(d["t"])
 
 
+def write_run_config(output_dir: str | os.PathLike, cfg, args: dict | None = None) -> Path:
+    """Record argv, the resolved config and the host beside the outputs,
+    never overwriting an earlier leg's record (the reference's
+    run_config.json layout, scripts/run_polar.py:233-268)."""
+    out = Path(output_dir)
+    out.mkdir(parents=True, exist_ok=True)
+
+    def plain(v):
+        return v if isinstance(v, (int, float, str, bool, type(None))) else str(v)
+
+    record = {
+        "argv": sys.argv,
+        "command": " ".join(shlex.quote(a) for a in [sys.executable, *sys.argv]),
+        "args": {k: plain(v) for k, v in (args or {}).items()},
+        "config": {f.name: plain(getattr(cfg, f.name)) for f in dataclasses.fields(cfg)},
+        "host": socket.gethostname(),
+        "python": platform.python_version(),
+        "backend": "nhqg_amd (MI355X, gfx950)",
+        "started_utc": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
+    }
+    path = out / "run_config.json"
+    if path.exists():
+        n = 1
+        while (out / f"run_config.{n}.json").exists():
+            n += 1
+        path = out / f"run_config.{n}.json"
+    path.write_text(json.dumps(record, indent=2, default=str) + "\n")
+    return path
+
+
+def read_run_config(path: str | os.PathLike):
+    """NHQGConfig from a run_config.json record (fields the config does not
+    know are ignored)."""
+    from nhqg_amd.config import NHQGConfig
+
+    with open(path) as f:
+        rec = json.load(f)
+    names = {f.name for f in dataclasses.fields(NHQGConfig)}
+    return NHQGConfig(**{k: v for k, v in rec["config"].items() if k in names})
+
+
 def diagnostics_finite(diag: dict) -> bool:
     return all(math.isfinite(float(diag[k])) for k in SCALAR_KEYS if k in diag)
 
@@ -85,6 +137,9 @@ def run_loop(grid, state: State, n_total: int, diag_steps: int, output_dir: str 
     that state is written first, as the reference driver does)."""
     out = Path(output_dir)
     out.mkdir(parents=True, exist_ok=True)
+    write_run_config(out, grid.cfg, {"n_total": n_total, "diag_steps": diag_steps,
+                                     "step0": step0, "t0": t0,
+                                     "checkpoint_steps": checkpoint_steps})
     csv_path = out / "diagnostics.csv"
     sp = Stepper(grid)
     sp.set_state(state)

@@ -88,3 +88,15 @@ def test_run_loop_stops_on_nonfinite(tmp_path):
     assert os.path.exists(os.path.join(tmp_path, "checkpoint_00000010.npz"))
     rows = open(os.path.join(tmp_path, "diagnostics.csv")).read().strip().splitlines()
     assert len(rows) == 2
+
+
+def test_run_config_roundtrip_and_no_clobber(tmp_path):
+    from nhqg_amd.io import read_run_config, write_run_config
+    from nhqg_amd.init import polar_config_256
+
+    cfg = polar_config_256(Nz=64)
+    p1 = write_run_config(tmp_path, cfg, {"steps": 10})
+    p2 = write_run_config(tmp_path, cfg)
+    assert p1.name == "run_config.json" and p2.name == "run_config.1.json"
+    assert read_run_config(p1) == cfg
+    assert read_run_config(p2) == cfg
