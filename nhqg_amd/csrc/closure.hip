@@ -223,73 +223,141 @@ int backward(nhqg_mlp* m, int batch, float* hist = nullptr) {
 // ---------------------------------------------------------------------------
 // Closure in the step: for every mode m and part (Re, Im), x_j = in_scale *
 // q[n0 + j][m] (j < 16), E_q[n0 + j][m] += out_scale * MLP(x)_j with the odd
-// (bias-free) network.  One thread per (mode, part); BF16 weights expanded
-// to FP32 in LDS (80 KiB, read as broadcasts); both hidden layers fully
-// unrolled so the activations stay in registers.
+// (bias-free) network, on the matrix cores: the three layers are GEMMs over
+// a batch of 16 columns per wavefront on v_mfma_f32_16x16x32_bf16 (BF16
+// weights, FP32 accumulation).  Lane layout (checked on the device): A row =
+// lane % 16, B column = lane % 16, k = 8 * (lane / 16) + e for the 8
+// elements e of a lane, D row = 4 * (lane / 16) + r.
+//
+// A layer's D tiles feed the next layer's B fragments without any lane
+// exchange: for k-step ks a lane's 8 elements are hidden units
+//   u(ks, g, e) = 32 ks + 4 g + e (e < 4),  32 ks + 16 + 4 g + (e - 4) (e >= 4)
+// of its own column (g = lane / 16), which are the values the lane holds in
+// D tiles 2 ks and 2 ks + 1.  The weight fragments are permuted to that k
+// order on the host (closure_export_bf16).  Activations enter the MFMAs as
+// a BF16 pair hi + lo (about 16 bits), close to the FP32 activations the
+// network was trained with.
 // ---------------------------------------------------------------------------
 #define CL_NIN 16
 #define CL_W 128
-__global__ void __launch_bounds__(256)
-closure_apply_kernel(int nm2, int ld, int n0, const __bf16* __restrict__ wb,
-                     const double* __restrict__ q, double* __restrict__ Eq, float in_scale,
-                     float out_scale) {
-  __shared__ float W1[CL_W * CL_NIN], W2[CL_W * CL_W], W3[CL_NIN * CL_W];
-  const int tid = threadIdx.x;
-  // flat order: W1, b1, W2, b2, W3, b3 (biases are zero and skipped)
-  constexpr int o_w2 = CL_W * CL_NIN + CL_W, o_w3 = o_w2 + CL_W * CL_W + CL_W;
-  for (int i = tid; i < CL_W * CL_NIN; i += 256) {
-    W1[i] = (float)wb[i];
-    W3[i] = (float)wb[o_w3 + i];
+// fragment buffer (bf16): W1 [8 tiles][64 lanes][8], W2 [8][4 k-steps][64][8],
+// W3 [4 k-steps][64][8]
+#define CL_F1 0
+#define CL_F2 (8 * 64 * 8)
+#define CL_F3 (CL_F2 + 8 * 4 * 64 * 8)
+#define CL_FN (CL_F3 + 4 * 64 * 8)
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+NHQG_DEV void split_bf16(const float (&v)[8], v8bf& hi, v8bf& lo) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const __bf16 h = (__bf16)v[e];
+    hi[e] = h;
+    lo[e] = (__bf16)(v[e] - (float)h);
   }
-  for (int i = tid; i < CL_W * CL_W; i += 256) W2[i] = (float)wb[o_w2 + i];
+}
+
+__global__ void __launch_bounds__(256)
+closure_mfma_kernel(int nm2, int ld, int n0, const v8bf* __restrict__ frag,
+                    const double* __restrict__ q, double* __restrict__ Eq, float in_scale,
+                    float out_scale) {
+  __shared__ v8bf F[CL_FN / 8];
+  for (int i = threadIdx.x; i < CL_FN / 8; i += 256) F[i] = frag[i];
   __syncthreads();
-  const int c = blockIdx.x * 256 + tid;   // column of the level-major field (2 m + part)
-  if (c >= nm2) return;
-  float x[CL_NIN];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4;
+  const int col = blockIdx.x * 64 + wave * 16 + (lane & 15);
+  const bool valid = col < nm2;
+  const int cc = valid ? col : nm2 - 1;
+  // layer 1 input: x_j, j = 8 g + e (g < 2), zero padding for g >= 2
+  float xv[8];
 #pragma unroll
-  for (int j = 0; j < CL_NIN; ++j) x[j] = in_scale * (float)q[(size_t)(n0 + j) * ld + c];
-  float h1[CL_W];
-#pragma unroll
-  for (int o = 0; o < CL_W; ++o) {
-    float s = 0.0f;
-#pragma unroll
-    for (int j = 0; j < CL_NIN; ++j) s = fmaf(W1[o * CL_NIN + j], x[j], s);
-    h1[o] = tanhf(s);
+  for (int e = 0; e < 8; ++e) {
+    const int j = min(8 * g + e, CL_NIN - 1);
+    const float x = in_scale * (float)q[(size_t)(n0 + j) * ld + cc];
+    xv[e] = g < 2 ? x : 0.0f;
   }
-  float y[CL_NIN];
+  v8bf bh, bl;
+  split_bf16(xv, bh, bl);
+  v4f h1[8];
 #pragma unroll
-  for (int j = 0; j < CL_NIN; ++j) y[j] = 0.0f;
-  for (int o = 0; o < CL_W; ++o) {
-    float s = 0.0f;
+  for (int mt = 0; mt < 8; ++mt) {
+    v4f a = {0.f, 0.f, 0.f, 0.f};
+    const v8bf w = F[(CL_F1 / 8) + mt * 64 + lane];
+    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, bh, a, 0, 0, 0);
+    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, bl, a, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < CL_W; ++i) s = fmaf(W2[o * CL_W + i], h1[i], s);
-    const float h = tanhf(s);
-#pragma unroll
-    for (int j = 0; j < CL_NIN; ++j) y[j] = fmaf(W3[j * CL_W + o], h, y[j]);
+    for (int r = 0; r < 4; ++r) h1[mt][r] = tanhf(a[r]);
   }
+  // layer 2: B fragments of the four k-steps from the layer-1 tiles
+  v8bf b2h[4], b2l[4];
 #pragma unroll
-  for (int j = 0; j < CL_NIN; ++j) Eq[(size_t)(n0 + j) * ld + c] += (double)(out_scale * y[j]);
+  for (int ks = 0; ks < 4; ++ks) {
+    float v[8];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[r] = h1[2 * ks][r];
+      v[4 + r] = h1[2 * ks + 1][r];
+    }
+    split_bf16(v, b2h[ks], b2l[ks]);
+  }
+  v4f h2[8];
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) {
+    v4f a = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const v8bf w = F[(CL_F2 / 8) + (mt * 4 + ks) * 64 + lane];
+      a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, b2h[ks], a, 0, 0, 0);
+      a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, b2l[ks], a, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h2[mt][r] = tanhf(a[r]);
+  }
+  // layer 3: one output tile, outputs 4 g + r of this lane's column
+  v4f y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    float v[8];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[r] = h2[2 * ks][r];
+      v[4 + r] = h2[2 * ks + 1][r];
+    }
+    v8bf ph, pl;
+    split_bf16(v, ph, pl);
+    const v8bf w = F[(CL_F3 / 8) + ks * 64 + lane];
+    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, ph, y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, pl, y, 0, 0, 0);
+  }
+  if (valid) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Eq[(size_t)(n0 + 4 * g + r) * ld + col] += (double)(out_scale * y[r]);
+  }
 }
 
-hipError_t launch_closure(int nm2, int ld, int n0, const void* wb, const double* q, double* Eq,
+hipError_t launch_closure(int nm2, int ld, int n0, const void* frag, const double* q, double* Eq,
                           float in_scale, float out_scale, hipStream_t s) {
-  hipLaunchKernelGGL(closure_apply_kernel, dim3((unsigned)((nm2 + 255) / 256)), dim3(256), 0, s,
-                     nm2, ld, n0, (const __bf16*)wb, q, Eq, in_scale, out_scale);
+  if (nm2 < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(closure_mfma_kernel, dim3((unsigned)((nm2 + 63) / 64)), dim3(256), 0, s, nm2,
+                     ld, n0, (const v8bf*)frag, q, Eq, in_scale, out_scale);
   return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(256)
-to_bf16_kernel(size_t n, const float* __restrict__ w, __bf16* __restrict__ out) {
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) out[i] = (__bf16)w[i];
+namespace {
+// float -> bf16 bits, round to nearest even (finite inputs)
+uint16_t to_bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
 }
+}  // namespace
 
-hipError_t launch_to_bf16(size_t n, const float* w, void* out, hipStream_t s) {
-  hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(n)), dim3(256), 0, s, n, w, (__bf16*)out);
-  return hipGetLastError();
-}
-
-// BF16 copy of the parameters for the step (shapes must be 16 -> 128 -> 16)
+// BF16 weight fragments of the step's closure in the kernel's LDS order
+// (shapes must be 16 -> 128 -> 128 -> 16, odd network)
 int closure_export_bf16(const nhqg_mlp* m, void** dst, hipStream_t s, std::string& err) {
   if (m->n_in != CL_NIN || m->width != CL_W || m->n_out != CL_NIN) {
     err = "the step closure needs a 16 -> 128 -> 128 -> 16 network";
@@ -299,13 +367,38 @@ int closure_export_bf16(const nhqg_mlp* m, void** dst, hipStream_t s, std::strin
     err = "the step closure needs an odd network (created with NHQG_MLP_ODD)";
     return NHQG_ERR_ARG;
   }
-  if (!*dst && hipMalloc(dst, m->n_par * sizeof(__bf16)) != hipSuccess) {
+  std::vector<float> w(m->n_par);
+  if (hipStreamSynchronize(m->st) != hipSuccess ||
+      hipMemcpy(w.data(), m->w, m->n_par * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+    err = "reading the closure weights failed";
+    return NHQG_ERR_HIP;
+  }
+  const float* W1 = w.data();
+  const float* W2 = W1 + CL_W * CL_NIN + CL_W;
+  const float* W3 = W2 + CL_W * CL_W + CL_W;
+  auto u = [](int ks, int g, int e) { return e < 4 ? 32 * ks + 4 * g + e : 32 * ks + 16 + 4 * g + e - 4; };
+  std::vector<uint16_t> f(CL_FN, 0);
+  for (int mt = 0; mt < 8; ++mt)
+    for (int lane = 0; lane < 64; ++lane)
+      for (int e = 0; e < 8; ++e) {
+        const int j = 8 * (lane / 16) + e;
+        f[CL_F1 + (mt * 64 + lane) * 8 + e] =
+            j < CL_NIN ? to_bf16_bits(W1[(16 * mt + lane % 16) * CL_NIN + j]) : 0;
+        for (int ks = 0; ks < 4; ++ks)
+          f[CL_F2 + ((mt * 4 + ks) * 64 + lane) * 8 + e] =
+              to_bf16_bits(W2[(16 * mt + lane % 16) * CL_W + u(ks, lane / 16, e)]);
+      }
+  for (int ks = 0; ks < 4; ++ks)
+    for (int lane = 0; lane < 64; ++lane)
+      for (int e = 0; e < 8; ++e)
+        f[CL_F3 + (ks * 64 + lane) * 8 + e] = to_bf16_bits(W3[(lane % 16) * CL_W + u(ks, lane / 16, e)]);
+  if (!*dst && hipMalloc(dst, CL_FN * sizeof(uint16_t)) != hipSuccess) {
     err = "hipMalloc failed for the closure weights";
     return NHQG_ERR_HIP;
   }
-  if (hipStreamSynchronize(m->st) != hipSuccess || launch_to_bf16(m->n_par, m->w, *dst, s) != hipSuccess ||
+  if (hipMemcpyAsync(*dst, f.data(), CL_FN * sizeof(uint16_t), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
-    err = "closure weight conversion failed";
+    err = "uploading the closure weights failed";
     return NHQG_ERR_HIP;
   }
   return NHQG_OK;
