@@ -400,6 +400,116 @@ row_flux(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double
 }
 
 // ---------------------------------------------------------------------------
+// Pass 2 (flux form), two rows per team: rows x and x+1 of one level share
+// every packed transform -- u, v, each field and each sponge input as
+// (row x) + i (row x+1) inverses, and the products as forward pairs.  The
+// two members of a pair are the same field at the same level, so they are
+// of equal scale and the unpacking's roundoff leak is harmless (as for u
+// with v); 17 complex transforms per two rows instead of 26.
+// ---------------------------------------------------------------------------
+template <int N, bool SPONGE, bool TRAP>
+__global__ void __launch_bounds__(256)
+row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double2* __restrict__ Y6,
+          const double* __restrict__ ky_col, const double* __restrict__ eta_phys,
+          const double* __restrict__ sponge_phys, const double2* __restrict__ twg) {
+  constexpr int T = N / 4, TEAMS = 256 / T;
+  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 tw[N];
+  const int tid = threadIdx.x, team = tid / T, t = tid % T;
+  const int l = blockIdx.y;
+  int x0 = 2 * (blockIdx.x * TEAMS + team);
+  const bool active = x0 < N;
+  if (!active) x0 = N - 2;  // duplicate valid rows; results are not stored
+  load_twiddles<N>(tw, twg);
+  __syncthreads();
+  double2* ba = buf[0][team];
+  double2* bb = buf[1][team];
+  const size_t r0 = ((size_t)l * N + x0) * nky, r1 = r0 + nky;
+  const double sc = 1.0 / ((double)N * (double)N);
+  double e0[4], e1[4], s0[4], s1[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const size_t y = t + j * T;
+    e0[j] = TRAP ? eta_phys[(size_t)x0 * N + y] : 0.0;
+    e1[j] = TRAP ? eta_phys[(size_t)(x0 + 1) * N + y] : 0.0;
+    s0[j] = SPONGE ? sponge_phys[(size_t)x0 * N + y] : 0.0;
+    s1[j] = SPONGE ? sponge_phys[(size_t)(x0 + 1) * N + y] : 0.0;
+  }
+  auto store_pair = [&](int slot, const double2 (&A)[3], const double2 (&B)[3], bool iky) {
+    if (!active) return;
+    double2* d0 = Y6 + slot * fstride + r0;
+    double2* d1 = Y6 + slot * fstride + r1;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int ky = t + q * T;
+      if (ky < nky) {
+        d0[ky] = iky ? cmul_i(A[q], ky_col[ky]) : A[q];
+        d1[ky] = iky ? cmul_i(B[q], ky_col[ky]) : B[q];
+      }
+    }
+  };
+  Half4<N> hc, hn;
+  load_half<N>(hc, t, nky, X5 + 0 * fstride + r0, X5 + 0 * fstride + r1);
+  load_half<N>(hn, t, nky, X5 + 1 * fstride + r0, X5 + 1 * fstride + r1);
+  double u0[4], u1[4], v0[4], v1[4];
+  inv_pair_regs<N>(ba, bb, t, tw, hc, u0, u1, sc);
+  hc = hn;
+  load_half<N>(hn, t, nky, X5 + 2 * fstride + r0, X5 + 2 * fstride + r1);
+  inv_pair_regs<N>(ba, bb, t, tw, hc, v0, v1, sc);
+#pragma unroll 1
+  for (int f = 0; f < 3; ++f) {
+    hc = hn;
+    if (f < 2) load_half<N>(hn, t, nky, X5 + (3 + f) * fstride + r0, X5 + (3 + f) * fstride + r1);
+    else if (SPONGE) load_half<N>(hn, t, nky, X5 + 5 * fstride + r0, X5 + 5 * fstride + r1);
+    double f0[4], f1[4];
+    inv_pair_regs<N>(ba, bb, t, tw, hc, f0, f1, sc);
+    if (TRAP && f == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        f0[j] += e0[j];
+        f1[j] += e1[j];
+      }
+    }
+    double a0[4], a1[4];
+    double2 A[3], B[3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a0[j] = u0[j] * f0[j];
+      a1[j] = u1[j] * f1[j];
+    }
+    fwd_pair<N>(ba, bb, t, tw, a0, a1, nky, A, B);
+    store_pair(2 * f, A, B, false);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a0[j] = v0[j] * f0[j];
+      a1[j] = v1[j] * f1[j];
+    }
+    fwd_pair<N>(ba, bb, t, tw, a0, a1, nky, A, B);
+    store_pair(2 * f + 1, A, B, true);
+  }
+  if (SPONGE) {
+    // sigma times each coefficient level of q', w, theta (spectral.py:53-60)
+#pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+      if (f > 0 && l >= ng) break;  // block-uniform
+      hc = hn;
+      if (f < 2 && l < ng)
+        load_half<N>(hn, t, nky, X5 + (6 + f) * fstride + r0, X5 + (6 + f) * fstride + r1);
+      double c0[4], c1[4], a0[4], a1[4];
+      inv_pair_regs<N>(ba, bb, t, tw, hc, c0, c1, sc);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a0[j] = s0[j] * c0[j];
+        a1[j] = s1[j] * c1[j];
+      }
+      double2 A[3], B[3];
+      fwd_pair<N>(ba, bb, t, tw, a0, a1, nky, A, B);
+      store_pair(6 + f, A, B, false);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Pass 2, Jacobian form (nhqg/spectral.py:281-304): per (level, x) row,
 // f_y = F_y^-1(i ky Q_f) and f_x = F_y^-1(iKx Q_f) as one packed inverse,
 // J_f = u f_x + v f_y (= psi_x f_y - psi_y f_x), and G_f = F_y(J_f) into
@@ -651,11 +761,12 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
       hipLaunchKernelGGL((row_jac<N, SP, false>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_x, a.eta_y, a.sponge_phys, a.tw);
   } else if (pass == 0 || pass == 2) {
+    dim3 g2((N / 2 + TEAMS - 1) / TEAMS, a.nq);   // two rows per team
     if (a.eta_phys)
-      hipLaunchKernelGGL((row_flux<N, SP, true>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
+      hipLaunchKernelGGL((row_flux2<N, SP, true>), g2, dim3(256), 0, s, a.nky, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
     else
-      hipLaunchKernelGGL((row_flux<N, SP, false>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
+      hipLaunchKernelGGL((row_flux2<N, SP, false>), g2, dim3(256), 0, s, a.nky, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
   }
   if (pass == 0 || pass == 3)
