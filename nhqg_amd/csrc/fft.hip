@@ -11,10 +11,11 @@
 //   col_inverse : per (level, ky): inverse DFT along x of
 //                 U = i ky psi, V = -i kx psi, Q, W, Theta (psi = -q / |k|^2
 //                 is formed on the fly from the nodal q)           -> X5
-//   row_flux    : per (level, x):  c2r along y (u and v packed in one
-//                 complex FFT), pointwise products u(q+eta), v(q+eta), uw,
-//                 vw, u theta, v theta, sigma q, sigma w, sigma theta,
-//                 r2c along y ((u f, v f) packed); applies i ky and the
+//   row_flux2   : per (level, row pair x, x+1): c2r along y (the two rows
+//                 packed in one complex FFT), pointwise products
+//                 u(q+eta), v(q+eta), uw, vw, u theta, v theta, sigma q,
+//                 sigma w, sigma theta, r2c along y (row pairs packed);
+//                 applies i ky and the
 //                 sponge                                           -> Y6
 //   col_forward : per (level, ky): forward DFT along x, i kx G + H,
 //                 gather of the kept rows (2/3 band)                -> A
@@ -304,108 +305,19 @@ NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, 
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2: per (level, x) row.  Y6[g][l][x][ky], g = Gq, Hq, Gw, Hw, Gth, Hth
-// with G_f = F_y(u f) and H_f = i ky F_y(v f) + F_y(sigma f).
-// ---------------------------------------------------------------------------
-template <int N, bool SPONGE, bool TRAP>
-__global__ void __launch_bounds__(256)
-row_flux(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double2* __restrict__ Y6,
-         const double* __restrict__ ky_col, const double* __restrict__ eta_phys,
-         const double* __restrict__ sponge_phys, const double2* __restrict__ twg) {
-  constexpr int T = N / 4, TEAMS = 256 / T;
-  __shared__ double2 buf[2][TEAMS][N];
-  __shared__ double2 tw[N];
-  const int tid = threadIdx.x, team = tid / T, t = tid % T;
-  const int l = blockIdx.y;
-  int x = blockIdx.x * TEAMS + team;
-  const bool active = x < N;
-  if (!active) x = N - 1;  // duplicate a valid row; results are not stored
-  load_twiddles<N>(tw, twg);
-  __syncthreads();
-  double2* ba = buf[0][team];
-  double2* bb = buf[1][team];
-  const size_t ro = ((size_t)l * N + x) * nky;
-  const double sc = 1.0 / ((double)N * (double)N);
-  // Only rows that end in the same output field share a packed FFT: the
-  // unpacking leaks roundoff of one member into the other, which is
-  // harmless at equal scale (u with v; u f with v f) but not across fields
-  // of very different size (a strong barotropic q next to a weak w).
-  // Each X5 row is loaded into registers one transform ahead of its use.
-  double eta[4], sig[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    eta[j] = TRAP ? eta_phys[(size_t)x * N + t + j * T] : 0.0;
-    sig[j] = SPONGE ? sponge_phys[(size_t)x * N + t + j * T] : 0.0;
-  }
-  Half4<N> hc, hn;
-  load_half<N>(hc, t, nky, X5 + 0 * fstride + ro, X5 + 1 * fstride + ro);
-  load_half<N>(hn, t, nky, X5 + 2 * fstride + ro, nullptr);
-  double u[4], v[4];
-  inv_pair_regs<N>(ba, bb, t, tw, hc, u, v, sc);
-#pragma unroll 1
-  for (int f = 0; f < 3; ++f) {
-    hc = hn;
-    if (f < 2) load_half<N>(hn, t, nky, X5 + (3 + f) * fstride + ro, nullptr);
-    else if (SPONGE) load_half<N>(hn, t, nky, X5 + 5 * fstride + ro, nullptr);
-    double fl[4];
-    inv_pair_regs<N>(ba, bb, t, tw, hc, fl, nullptr, sc);
-    if (TRAP && f == 0) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) fl[j] += eta[j];
-    }
-    double a[4], b[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      a[j] = u[j] * fl[j];
-      b[j] = v[j] * fl[j];
-    }
-    double2 G[3], Hv[3];
-    fwd_pair<N>(ba, bb, t, tw, a, b, nky, G, Hv);
-    if (active) {
-      double2* dg = Y6 + (2 * f) * fstride + ro;
-      double2* dh = Y6 + (2 * f + 1) * fstride + ro;
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        const int ky = t + s * T;
-        if (ky < nky) {
-          dg[ky] = G[s];
-          dh[ky] = cmul_i(Hv[s], ky_col[ky]);
-        }
-      }
-    }
-  }
-  if (SPONGE) {
-    // sigma times each coefficient level of q', w, theta (spectral.py:53-60)
-#pragma unroll 1
-    for (int f = 0; f < 3; ++f) {
-      if (f > 0 && l >= ng) break;  // block-uniform
-      hc = hn;
-      if (f < 2 && l < ng) load_half<N>(hn, t, nky, X5 + (6 + f) * fstride + ro, nullptr);
-      double c[4], a[4];
-      inv_pair_regs<N>(ba, bb, t, tw, hc, c, nullptr, sc);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) a[j] = sig[j] * c[j];
-      double2 S[3], dummy[3];
-      fwd_pair<N>(ba, bb, t, tw, a, nullptr, nky, S, dummy);
-      if (active) {
-        double2* ds = Y6 + (6 + f) * fstride + ro;
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          const int ky = t + s * T;
-          if (ky < nky) ds[ky] = S[s];
-        }
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Pass 2 (flux form), two rows per team: rows x and x+1 of one level share
-// every packed transform -- u, v, each field and each sponge input as
-// (row x) + i (row x+1) inverses, and the products as forward pairs.  The
-// two members of a pair are the same field at the same level, so they are
-// of equal scale and the unpacking's roundoff leak is harmless (as for u
-// with v); 17 complex transforms per two rows instead of 26.
+// Pass 2 (flux form): Y6[g][l][x][ky], g = Gq, Hq, Gw, Hw, Gth, Hth with
+// G_f = F_y(u f) and H_f = i ky F_y(v f) + F_y(sigma f).  Two rows per team:
+// rows x and x+1 of one level share every packed transform -- u, v, each
+// field and each sponge input as (row x) + i (row x+1) inverses, and the
+// products as forward pairs.  The two members of a pair are the same field
+// at the same level, so they are of equal scale and the unpacking's
+// roundoff leak is harmless.  Fields are never packed with each other: the
+// leak across fields of very different size (a strong barotropic q next to
+// a weak w) is not.  17 complex transforms per two rows instead of the 26 of
+// a one-row pass that packs u with v and u f with v f.  Each half-spectrum
+// is loaded just before its transform: loading one transform ahead costs
+// 44 VGPRs and drops occupancy from 3 to 2 waves per SIMD, which loses more
+// than the hidden load latency gains.
 // ---------------------------------------------------------------------------
 template <int N, bool SPONGE, bool TRAP>
 __global__ void __launch_bounds__(256)
@@ -448,19 +360,15 @@ row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, doubl
       }
     }
   };
-  Half4<N> hc, hn;
+  Half4<N> hc;
   load_half<N>(hc, t, nky, X5 + 0 * fstride + r0, X5 + 0 * fstride + r1);
-  load_half<N>(hn, t, nky, X5 + 1 * fstride + r0, X5 + 1 * fstride + r1);
   double u0[4], u1[4], v0[4], v1[4];
   inv_pair_regs<N>(ba, bb, t, tw, hc, u0, u1, sc);
-  hc = hn;
-  load_half<N>(hn, t, nky, X5 + 2 * fstride + r0, X5 + 2 * fstride + r1);
+  load_half<N>(hc, t, nky, X5 + 1 * fstride + r0, X5 + 1 * fstride + r1);
   inv_pair_regs<N>(ba, bb, t, tw, hc, v0, v1, sc);
 #pragma unroll 1
   for (int f = 0; f < 3; ++f) {
-    hc = hn;
-    if (f < 2) load_half<N>(hn, t, nky, X5 + (3 + f) * fstride + r0, X5 + (3 + f) * fstride + r1);
-    else if (SPONGE) load_half<N>(hn, t, nky, X5 + 5 * fstride + r0, X5 + 5 * fstride + r1);
+    load_half<N>(hc, t, nky, X5 + (2 + f) * fstride + r0, X5 + (2 + f) * fstride + r1);
     double f0[4], f1[4];
     inv_pair_regs<N>(ba, bb, t, tw, hc, f0, f1, sc);
     if (TRAP && f == 0) {
@@ -492,9 +400,7 @@ row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, doubl
 #pragma unroll 1
     for (int f = 0; f < 3; ++f) {
       if (f > 0 && l >= ng) break;  // block-uniform
-      hc = hn;
-      if (f < 2 && l < ng)
-        load_half<N>(hn, t, nky, X5 + (6 + f) * fstride + r0, X5 + (6 + f) * fstride + r1);
+      load_half<N>(hc, t, nky, X5 + (5 + f) * fstride + r0, X5 + (5 + f) * fstride + r1);
       double c0[4], c1[4], a0[4], a1[4];
       inv_pair_regs<N>(ba, bb, t, tw, hc, c0, c1, sc);
 #pragma unroll
