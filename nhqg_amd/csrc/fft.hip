@@ -45,6 +45,18 @@ struct ColBlock {
   static constexpr int v = (N >= 512) ? 1024 : 256;
 };
 
+// LDS layout of a team's transform buffer.  Element i of a team's N-point
+// buffer sits at swz(i): bits 2..5 are XORed into bits 0..3 (a bijection on
+// every aligned block of 64, the identity above bit 5).  The radix-4
+// stages write at strides 4 and in runs of 4 spaced 16 apart, and the
+// column passes stash and gather across teams; with the plain layout those
+// accesses put 4 lanes of one 16-byte phase on the same banks.  With swz and
+// team buffers FFT_PAD elements longer than N (so that neighbouring teams
+// start on different banks) every such access is conflict-free or within
+// 1.1x of it, for 8 or 16 lanes per 16-byte LDS phase, N = 16..1024.
+constexpr int FFT_PAD = 2;
+NHQG_DEV int swz(int i) { return i ^ ((i >> 2) & 15); }
+
 template <bool INV>
 NHQG_DEV double2 twid(const double2* tw, int e) {
   const double2 w = tw[e];
@@ -66,8 +78,9 @@ NHQG_DEV void team_sync() {
   }
 }
 
-// Unnormalised DFT of N points held in LDS buffer a (b is scratch), by a
-// team of N/4 threads with index t.  INV selects exp(+i...).  Every thread
+// Unnormalised DFT of N points held in LDS buffer a (b is scratch; both in
+// the swz layout), by a team of N/4 threads with index t, twiddles from the
+// stage-major table of load_twiddles.  INV selects exp(+i...).  Every thread
 // of the block must call it (team barriers inside; a caller that reads
 // another team's result after it needs a block barrier).  Returns the
 // buffer that holds the result in natural order.
@@ -82,21 +95,23 @@ NHQG_DEV double2* team_fft(double2* a, double2* b, int t, const double2* tw) {
   for (int s = 0; s < L2 / 2; ++s) {
     const int j = t;
     const int k = j & (Ns - 1);
-    double2 v0 = in[j], v1 = in[j + T], v2 = in[j + 2 * T], v3 = in[j + 3 * T];
+    double2 v0 = in[swz(j)], v1 = in[swz(j + T)], v2 = in[swz(j + 2 * T)],
+            v3 = in[swz(j + 3 * T)];
     if (s > 0) {
-      const int e = k * (N / (Ns * 4));
-      v1 = cmul(v1, twid<INV>(tw, e));
-      v2 = cmul(v2, twid<INV>(tw, 2 * e));
-      v3 = cmul(v3, twid<INV>(tw, 3 * e));
+      // stage-major table: w^(p k N / (4 Ns)) at tw[4^s - 4 + (p - 1) Ns + k]
+      const double2* ts = tw + (Ns - 4);
+      v1 = cmul(v1, twid<INV>(ts, k));
+      v2 = cmul(v2, twid<INV>(ts, Ns + k));
+      v3 = cmul(v3, twid<INV>(ts, 2 * Ns + k));
     }
     const double2 a0 = cadd(v0, v2), a1 = csub(v0, v2);
     const double2 a2 = cadd(v1, v3), a3 = csub(v1, v3);
     const double2 ia3 = make_double2(-a3.y, a3.x);
     const int d = (j - k) * 4 + k;
-    out[d] = cadd(a0, a2);
-    out[d + Ns] = INV ? cadd(a1, ia3) : csub(a1, ia3);
-    out[d + 2 * Ns] = csub(a0, a2);
-    out[d + 3 * Ns] = INV ? csub(a1, ia3) : cadd(a1, ia3);
+    out[swz(d)] = cadd(a0, a2);
+    out[swz(d + Ns)] = INV ? cadd(a1, ia3) : csub(a1, ia3);
+    out[swz(d + 2 * Ns)] = csub(a0, a2);
+    out[swz(d + 3 * Ns)] = INV ? csub(a1, ia3) : cadd(a1, ia3);
     team_sync<T>();
     double2* tmp = in;
     in = out;
@@ -108,11 +123,12 @@ NHQG_DEV double2* team_fft(double2* a, double2* b, int t, const double2* tw) {
     for (int h = 0; h < 2; ++h) {
       const int j = t + h * T;
       const int k = j & (Ns - 1);
-      const double2 v0 = in[j];
-      const double2 v1 = cmul(in[j + N / 2], twid<INV>(tw, k * (N / (Ns * 2))));
+      const double2 v0 = in[swz(j)];
+      // radix-2 table: w^(k N / (2 Ns)) at tw[Ns - 4 + k]
+      const double2 v1 = cmul(in[swz(j + N / 2)], twid<INV>(tw + (Ns - 4), k));
       const int d = (j - k) * 2 + k;
-      out[d] = cadd(v0, v1);
-      out[d + Ns] = csub(v0, v1);
+      out[swz(d)] = cadd(v0, v1);
+      out[swz(d + Ns)] = csub(v0, v1);
     }
     team_sync<T>();
     double2* tmp = in;
@@ -122,9 +138,43 @@ NHQG_DEV double2* team_fft(double2* a, double2* b, int t, const double2* tw) {
   return in;
 }
 
+// Stage-major twiddle table from the natural one (twg[e] = exp(-2 pi i e/N)):
+// radix-4 stage s >= 1 (Ns = 4^s) holds w^(p k N / (4 Ns)) for p = 1, 2, 3
+// and k < Ns at 4^s - 4 + (p - 1) Ns + k; a final radix-2 stage (odd log2 N)
+// holds w^(k N / (2 Ns)) at Ns - 4 + k.  A stage's lanes read consecutive
+// (or broadcast) entries, where strided reads of the natural table put up to
+// 4 lanes on one bank.  At most N - 4 entries in all.
+// Workgroups are handed to the 8 XCDs round-robin by linear id, each XCD
+// with its own L2.  The column passes run on a 1-D grid of 8 * per
+// workgroups (xcd_grid) remapped here so that XCD x takes the contiguous
+// logical range [x * per, (x + 1) * per), logical = column block + gx *
+// level: neighbouring column blocks, whose pieces of every X5 / Y6 row share
+// cache lines, then meet in one L2 instead of each XCD fetching the lines
+// from HBM.  Returns false for the padding workgroups (logical >= nb).
+NHQG_DEV bool xcd_column_block(int gx, int nb, int& bx, int& l) {
+  const int per = gridDim.x >> 3;
+  const int b = blockIdx.x;
+  const int logical = (b & 7) * per + (b >> 3);
+  if (logical >= nb) return false;
+  bx = logical % gx;
+  l = logical / gx;
+  return true;
+}
+
 template <int N>
 NHQG_DEV void load_twiddles(double2* tw, const double2* twg) {
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
+  constexpr int L2 = Log2<N>::v;
+  int Ns = 4;
+#pragma unroll
+  for (int s = 1; s < L2 / 2; ++s) {
+    for (int i = threadIdx.x; i < 3 * Ns; i += blockDim.x) {
+      const int p = i / Ns + 1, k = i % Ns;
+      tw[Ns - 4 + i] = twg[p * k * (N / (4 * Ns))];
+    }
+    Ns *= 4;
+  }
+  if (L2 & 1)
+    for (int i = threadIdx.x; i < Ns; i += blockDim.x) tw[Ns - 4 + i] = twg[i * (N / (2 * Ns))];
 }
 
 // ---------------------------------------------------------------------------
@@ -143,16 +193,17 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
                  const double2* __restrict__ thc, const int* __restrict__ kidx_of_x,
                  const double* __restrict__ kx_row, const double* __restrict__ ky_col,
                  const double* __restrict__ inv_denom, const double2* __restrict__ twg,
-                 double2* __restrict__ X5, size_t fstride) {
+                 double2* __restrict__ X5, size_t fstride, int nb) {
   constexpr int BS = ColBlock<N>::v;
   constexpr int T = N / 4, TEAMS = BS / T;
   constexpr int NA = JAC ? 8 : 5;   // advection inputs
   constexpr int NF = NA + (SPONGE ? 3 : 0);
-  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 buf[2][TEAMS][N + FFT_PAD];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
-  const int l = blockIdx.y;
-  const int ky0 = blockIdx.x * TEAMS;
+  int bx, l;
+  if (!xcd_column_block((nky + TEAMS - 1) / TEAMS, nb, bx, l)) return;   // whole workgroup
+  const int ky0 = bx * TEAMS;
   const int ky = ky0 + team;
   const bool active = ky < nky;
   load_twiddles<N>(tw, twg);
@@ -177,7 +228,7 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
           v = (f == NA ? qc : f == NA + 1 ? wc : thc)[lb + m];
         }
       }
-      buf[0][team][x] = v;
+      buf[0][team][swz(x)] = v;
     }
     __syncthreads();
     double2* res = team_fft<N, true>(buf[0][team], buf[1][team], t, tw);
@@ -187,7 +238,7 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
     for (int idx = tid; idx < N * TEAMS; idx += BS) {
       const int cc = idx % TEAMS, x = idx / TEAMS;
       const int kyc = ky0 + cc;
-      if (kyc < nky) dst[(size_t)x * nky + kyc] = buf[par][cc][x];
+      if (kyc < nky) dst[(size_t)x * nky + kyc] = buf[par][cc][swz(x)];
     }
     __syncthreads();
   }
@@ -211,7 +262,7 @@ NHQG_DEV void build_packed(double2* z, int t, int nky, const double2* P, const d
       else if (k < N / 2) v = make_double2(p.x - q.y, p.y + q.x);
       else v = make_double2(p.x + q.y, q.x - p.y);
     }
-    z[k] = v;
+    z[swz(k)] = v;
   }
 }
 
@@ -222,15 +273,15 @@ NHQG_DEV void fwd_pair(double2* ba, double2* bb, int t, const double2* tw, const
                        const double* b, int nky, double2* A, double2* B) {
   constexpr int T = N / 4;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) ba[t + j * T] = make_double2(a[j], b ? b[j] : 0.0);
+  for (int j = 0; j < 4; ++j) ba[swz(t + j * T)] = make_double2(a[j], b ? b[j] : 0.0);
   team_sync<T>();
   const double2* r = team_fft<N, false>(ba, bb, t, tw);
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const int ky = t + s * T;
     if (ky < nky) {
-      const double2 zk = r[ky];
-      const double2 zm = r[(N - ky) & (N - 1)];
+      const double2 zk = r[swz(ky)];
+      const double2 zm = r[swz((N - ky) & (N - 1))];
       A[s] = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
       B[s] = make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
     } else {
@@ -250,7 +301,7 @@ NHQG_DEV void inv_pair(double2* ba, double2* bb, int t, const double2* tw, int n
   const double2* r = team_fft<N, true>(ba, bb, t, tw);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const double2 z = r[t + j * T];
+    const double2 z = r[swz(t + j * T)];
     p[j] = z.x * scale;
     if (q) q[j] = z.y * scale;
   }
@@ -291,13 +342,13 @@ NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, 
     if (k == 0 || k == N / 2) v = make_double2(a.x, b.x);
     else if (k < N / 2) v = make_double2(a.x - b.y, a.y + b.x);
     else v = make_double2(a.x + b.y, b.x - a.y);
-    ba[k] = v;
+    ba[swz(k)] = v;
   }
   team_sync<T>();
   const double2* r = team_fft<N, true>(ba, bb, t, tw);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const double2 z = r[t + j * T];
+    const double2 z = r[swz(t + j * T)];
     p[j] = z.x * scale;
     if (q) q[j] = z.y * scale;
   }
@@ -325,7 +376,7 @@ row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, doubl
           const double* __restrict__ ky_col, const double* __restrict__ eta_phys,
           const double* __restrict__ sponge_phys, const double2* __restrict__ twg) {
   constexpr int T = N / 4, TEAMS = 256 / T;
-  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 buf[2][TEAMS][N + FFT_PAD];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
   const int l = blockIdx.y;
@@ -429,7 +480,7 @@ row_jac(int nky, int ng, const double2* __restrict__ X, size_t fstride, double2*
         const double* __restrict__ eta_y, const double* __restrict__ sponge_phys,
         const double2* __restrict__ twg) {
   constexpr int T = N / 4, TEAMS = 256 / T;
-  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 buf[2][TEAMS][N + FFT_PAD];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
   const int l = blockIdx.y;
@@ -527,14 +578,15 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
                  const double* __restrict__ mask, double2* __restrict__ out_q,
                  double2* __restrict__ out_w, double2* __restrict__ out_th,
                  double2* __restrict__ sp_q, double2* __restrict__ sp_w,
-                 double2* __restrict__ sp_th, const double2* __restrict__ twg) {
+                 double2* __restrict__ sp_th, const double2* __restrict__ twg, int nb) {
   constexpr int BS = ColBlock<N>::v;
   constexpr int T = N / 4, TEAMS = BS / T;
-  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 buf[2][TEAMS][N + FFT_PAD];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
-  const int l = blockIdx.y;
-  const int ky0 = blockIdx.x * TEAMS;
+  int bx, l;
+  if (!xcd_column_block((nky + TEAMS - 1) / TEAMS, nb, bx, l)) return;   // whole workgroup
+  const int ky0 = bx * TEAMS;
   const int ky = ky0 + team;
   const bool active = ky < nky;
   load_twiddles<N>(tw, twg);
@@ -583,7 +635,7 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
 #pragma unroll
     for (int it = 0; it < NL; ++it) {
       const int idx = tid + it * BS;
-      buf[0][idx % TEAMS][idx / TEAMS] = cur[it];
+      buf[0][idx % TEAMS][swz(idx / TEAMS)] = cur[it];
     }
     __syncthreads();
     const double2* r = team_fft<N, false>(buf[0][team], buf[1][team], t, tw);
@@ -591,7 +643,7 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
     for (int j = 0; j < 4; ++j) {
       const int i = t + j * T;
       if (i < nkx) {
-        const double2 z = r[rowv[j]];
+        const double2 z = r[swz(rowv[j])];
         const int m = ky * nkx + i;
         if (JAC) {
           if (active) (h == 0 ? outs : sps)[f][(size_t)l * ldc + m] = cscale(z, mkv[j]);
@@ -618,7 +670,7 @@ __global__ void __launch_bounds__(256)
 row_speed(int nky, const double2* __restrict__ X5, size_t fstride, double* __restrict__ rowmax,
           const double2* __restrict__ twg) {
   constexpr int T = N / 4, TEAMS = 256 / T;
-  __shared__ double2 buf[2][TEAMS][N];
+  __shared__ double2 buf[2][TEAMS][N + FFT_PAD];
   __shared__ double2 tw[N];
   __shared__ double red[256];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
@@ -649,16 +701,20 @@ row_speed(int nky, const double2* __restrict__ X5, size_t fstride, double* __res
 // ---------------------------------------------------------------------------
 
 
+// 1-D grid for xcd_column_block: nb logical workgroups padded to 8 per XCD
+static dim3 xcd_grid(int nb) { return dim3(8 * ((nb + 7) / 8)); }
+
 template <int N, bool SP, bool JAC>
 static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;                       // row passes
   constexpr int BC = ColBlock<N>::v, TC = 4 * BC / N;   // column passes
-  dim3 gc((a.nky + TC - 1) / TC, a.nq);
+  const int nb = (a.nky + TC - 1) / TC * a.nq;
+  const dim3 gc = xcd_grid(nb);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
   if (pass == 0 || pass == 1)
     hipLaunchKernelGGL((col_inverse_flux<N, SP, JAC>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc,
                        a.ng, a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row, a.ky_col,
-                       a.inv_denom, a.tw, a.X5, a.fstride);
+                       a.inv_denom, a.tw, a.X5, a.fstride, nb);
   if ((pass == 0 || pass == 2) && JAC) {
     if (a.eta_x)
       hipLaunchKernelGGL((row_jac<N, SP, true>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
@@ -678,7 +734,7 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
   if (pass == 0 || pass == 3)
     hipLaunchKernelGGL((col_forward_flux<N, SP, JAC>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc,
                        a.ng, a.Y6, a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath, a.sq,
-                       a.sw, a.sth, a.tw);
+                       a.sw, a.sth, a.tw, nb);
 }
 
 template <int N>
@@ -710,10 +766,10 @@ template <int N>
 static hipError_t speed_n(const FluxArgs& a, double* rowmax, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;
   constexpr int BC = ColBlock<N>::v, TC = 4 * BC / N;
-  dim3 gc((a.nky + TC - 1) / TC, a.nq);
-  hipLaunchKernelGGL((col_inverse_flux<N, false, false>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc, 0,
-                     a.qn, a.qn, a.qn, a.qn, a.qn, a.qn, a.kidx_of_x, a.kx_row, a.ky_col,
-                     a.inv_denom, a.tw, a.X5, a.fstride);
+  const int nb = (a.nky + TC - 1) / TC * a.nq;
+  hipLaunchKernelGGL((col_inverse_flux<N, false, false>), xcd_grid(nb), dim3(BC), 0, s, a.nkx,
+                     a.nky, a.ldc, 0, a.qn, a.qn, a.qn, a.qn, a.qn, a.qn, a.kidx_of_x, a.kx_row,
+                     a.ky_col, a.inv_denom, a.tw, a.X5, a.fstride, nb);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
   hipLaunchKernelGGL(row_speed<N>, gr, dim3(256), 0, s, a.nky, a.X5, a.fstride, rowmax, a.tw);
   return hipGetLastError();
