@@ -208,27 +208,48 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
   const bool active = ky < nky;
   load_twiddles<N>(tw, twg);
   const size_t lb = (size_t)l * ldc;
+  // mode index, kx and 1/|k|^2 of this thread's 4 points, the same for every
+  // field (-1: x outside the kept rows, or an idle team)
+  int mi[4];
+  double kxv[4], idn[4];
+  const double kyv = active ? ky_col[ky] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int i = kidx_of_x[t + j * T];
+    mi[j] = (active && i >= 0) ? ky * nkx + i : -1;
+    kxv[j] = mi[j] >= 0 ? kx_row[i] : 0.0;
+    idn[j] = mi[j] >= 0 ? inv_denom[mi[j]] : 0.0;
+  }
+  // source of field f: 0..2 = nodal q, w, theta; 3..5 = coefficient q, w, theta
+  auto src_of = [&](int f) { return f < 2 ? 0 : f < NA ? (JAC ? (f - 2) / 2 : f - 2) : 3 + f - NA; };
+  auto load = [&](int g, double2 (&v)[4]) {
+    const double2* P = g == 0 ? qn : g == 1 ? wn : g == 2 ? thn : g == 3 ? qc : g == 4 ? wc : thc;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = mi[j] >= 0 ? P[lb + mi[j]] : make_double2(0.0, 0.0);
+  };
+  // w and theta coefficient fields have ng < Nz+1 levels (block-uniform)
+  auto valid = [&](int f) { return !(f >= NA + 1 && l >= ng); };
+  // The next field's column is loaded into registers before this field's
+  // transform and stores, so its load latency overlaps them; fields with
+  // the same source (U, V and Q from q; the Jacobian form's f and i kx f)
+  // share one load.
+  double2 craw[4], nraw[4];
+  load(src_of(0), nraw);
+#pragma unroll 1
   for (int f = 0; f < NF; ++f) {
-    // w and theta coefficient fields have ng < Nz+1 levels (block-uniform)
-    if (f >= NA + 1 && l >= ng) continue;
-    for (int x = t; x < N; x += T) {
-      double2 v = make_double2(0.0, 0.0);
-      const int i = kidx_of_x[x];
-      if (active && i >= 0) {
-        const int m = ky * nkx + i;
-        if (f < 2) {
-          const double2 qv = qn[lb + m];
-          if (f == 0) v = cmul_i(qv, ky_col[ky] * inv_denom[m]);
-          else v = cmul_i(qv, -kx_row[i] * inv_denom[m]);
-        } else if (f < NA) {
-          const int g = JAC ? (f - 2) / 2 : f - 2;   // advected field q, w, theta
-          v = (g == 0 ? qn : g == 1 ? wn : thn)[lb + m];
-          if (JAC && (f - 2) % 2 == 1) v = cmul_i(v, kx_row[i]);
-        } else {
-          v = (f == NA ? qc : f == NA + 1 ? wc : thc)[lb + m];
-        }
-      }
-      buf[0][team][swz(x)] = v;
+    if (!valid(f)) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) craw[j] = nraw[j];
+    int fn = f + 1;
+    while (fn < NF && !valid(fn)) ++fn;
+    if (fn < NF && src_of(fn) != src_of(f)) load(src_of(fn), nraw);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double2 v = craw[j];
+      if (f == 0) v = cmul_i(v, kyv * idn[j]);
+      else if (f == 1) v = cmul_i(v, -kxv[j] * idn[j]);
+      else if (JAC && f < NA && (f - 2) % 2 == 1) v = cmul_i(v, kxv[j]);
+      buf[0][team][swz(t + j * T)] = v;
     }
     __syncthreads();
     double2* res = team_fft<N, true>(buf[0][team], buf[1][team], t, tw);
