@@ -144,6 +144,60 @@ NHQG_DEV double2* team_fft(double2* a, double2* b, int t, const double2* tw) {
 // holds w^(k N / (2 Ns)) at Ns - 4 + k.  A stage's lanes read consecutive
 // (or broadcast) entries, where strided reads of the natural table put up to
 // 4 lanes on one bank.  At most N - 4 entries in all.
+// The same transform for even log2 N with the data in registers at both
+// ends: v[m] holds point t + m*N/4 on entry and bin t + m*N/4 on return.
+// Those are exactly the points the first radix-4 stage reads and the bins
+// the last one writes in the thread itself, so the LDS write before the
+// first stage and the LDS read after the last are skipped (16 of the 32
+// element accesses per thread at N = 256).  a and b are scratch; ends with a
+// team barrier, so the buffers may be reused at once.
+template <int N, bool INV>
+NHQG_DEV void team_fft_reg(double2 (&v)[4], double2* a, double2* b, int t, const double2* tw) {
+  constexpr int T = N / 4;
+  constexpr int S = Log2<N>::v / 2;
+  static_assert(Log2<N>::v % 2 == 0, "register-ended transform needs even log2 N");
+  double2* in = a;
+  double2* out = b;
+  int Ns = 1;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int j = t;
+    const int k = j & (Ns - 1);
+    double2 v0, v1, v2, v3;
+    if (s == 0) {
+      v0 = v[0]; v1 = v[1]; v2 = v[2]; v3 = v[3];
+    } else {
+      v0 = in[swz(j)]; v1 = in[swz(j + T)]; v2 = in[swz(j + 2 * T)]; v3 = in[swz(j + 3 * T)];
+      const double2* ts = tw + (Ns - 4);
+      v1 = cmul(v1, twid<INV>(ts, k));
+      v2 = cmul(v2, twid<INV>(ts, Ns + k));
+      v3 = cmul(v3, twid<INV>(ts, 2 * Ns + k));
+    }
+    const double2 a0 = cadd(v0, v2), a1 = csub(v0, v2);
+    const double2 a2 = cadd(v1, v3), a3 = csub(v1, v3);
+    const double2 ia3 = make_double2(-a3.y, a3.x);
+    const double2 r0 = cadd(a0, a2), r2 = csub(a0, a2);
+    const double2 r1 = INV ? cadd(a1, ia3) : csub(a1, ia3);
+    const double2 r3 = INV ? csub(a1, ia3) : cadd(a1, ia3);
+    if (s == S - 1) {
+      // last stage: Ns = T, k = j, so its bins are d + m*Ns = t + m*T
+      v[0] = r0; v[1] = r1; v[2] = r2; v[3] = r3;
+    } else {
+      const int d = (j - k) * 4 + k;
+      out[swz(d)] = r0;
+      out[swz(d + Ns)] = r1;
+      out[swz(d + 2 * Ns)] = r2;
+      out[swz(d + 3 * Ns)] = r3;
+      team_sync<T>();
+      double2* tmp = in;
+      in = out;
+      out = tmp;
+    }
+    Ns *= 4;
+  }
+  team_sync<T>();
+}
+
 // Workgroups are handed to the 8 XCDs round-robin by linear id, each XCD
 // with its own L2.  The column passes run on a 1-D grid of 8 * per
 // workgroups (xcd_grid) remapped here so that XCD x takes the contiguous
@@ -293,10 +347,24 @@ template <int N>
 NHQG_DEV void fwd_pair(double2* ba, double2* bb, int t, const double2* tw, const double* a,
                        const double* b, int nky, double2* A, double2* B) {
   constexpr int T = N / 4;
+  const double2* r;
+  if constexpr (Log2<N>::v % 2 == 0) {
+    // the bins of the thread come back in registers; the pairing with bin
+    // N - ky needs every bin of the team, through LDS
+    double2 v[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) ba[swz(t + j * T)] = make_double2(a[j], b ? b[j] : 0.0);
-  team_sync<T>();
-  const double2* r = team_fft<N, false>(ba, bb, t, tw);
+    for (int j = 0; j < 4; ++j) v[j] = make_double2(a[j], b ? b[j] : 0.0);
+    team_fft_reg<N, false>(v, ba, bb, t, tw);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ba[swz(t + j * T)] = v[j];
+    team_sync<T>();
+    r = ba;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ba[swz(t + j * T)] = make_double2(a[j], b ? b[j] : 0.0);
+    team_sync<T>();
+    r = team_fft<N, false>(ba, bb, t, tw);
+  }
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const int ky = t + s * T;
@@ -355,25 +423,31 @@ template <int N>
 NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, const Half4<N>& h,
                             double* p, double* q, double scale) {
   constexpr int T = N / 4;
+  double2 v[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int k = t + j * T;
     const double2 a = h.p[j], b = h.q[j];
-    double2 v;
-    if (k == 0 || k == N / 2) v = make_double2(a.x, b.x);
-    else if (k < N / 2) v = make_double2(a.x - b.y, a.y + b.x);
-    else v = make_double2(a.x + b.y, b.x - a.y);
-    ba[swz(k)] = v;
+    if (k == 0 || k == N / 2) v[j] = make_double2(a.x, b.x);
+    else if (k < N / 2) v[j] = make_double2(a.x - b.y, a.y + b.x);
+    else v[j] = make_double2(a.x + b.y, b.x - a.y);
   }
-  team_sync<T>();
-  const double2* r = team_fft<N, true>(ba, bb, t, tw);
+  if constexpr (Log2<N>::v % 2 == 0) {
+    team_fft_reg<N, true>(v, ba, bb, t, tw);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ba[swz(t + j * T)] = v[j];
+    team_sync<T>();
+    const double2* r = team_fft<N, true>(ba, bb, t, tw);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = r[swz(t + j * T)];
+    team_sync<T>();
+  }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const double2 z = r[swz(t + j * T)];
-    p[j] = z.x * scale;
-    if (q) q[j] = z.y * scale;
+    p[j] = v[j].x * scale;
+    if (q) q[j] = v[j].y * scale;
   }
-  team_sync<T>();
 }
 
 // ---------------------------------------------------------------------------
@@ -389,10 +463,14 @@ NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, 
 // a one-row pass that packs u with v and u f with v f.  Each half-spectrum
 // is loaded just before its transform: loading one transform ahead costs
 // 44 VGPRs and drops occupancy from 3 to 2 waves per SIMD, which loses more
-// than the hidden load latency gains.
+// than the hidden load latency gains.  The pass is bound by LDS bandwidth;
+// its transforms start and end in registers (team_fft_reg) where log2 N is
+// even.  That needs a few more VGPRs than 3 waves per SIMD allow, and the
+// kernel is held at 3 (amdgpu_waves_per_eu) at the cost of a few spilled
+// registers in the trap variant: 15% faster than 2 waves without spills.
 // ---------------------------------------------------------------------------
 template <int N, bool SPONGE, bool TRAP>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double2* __restrict__ Y6,
           const double* __restrict__ ky_col, const double* __restrict__ eta_phys,
           const double* __restrict__ sponge_phys, const double2* __restrict__ twg) {
