@@ -56,6 +56,12 @@ typedef struct {
                          * FP64 reconstruction, this many 7-bit slices per operand; the
                          * SBP exchange, the k = 0 equation, FFT products and
                          * diagnostics stay FP64 */
+  int32_t dense_contractions; /* 0: a vertical operator with the reflection symmetry of the
+                               * Chebyshev grid (z -> -z: parity blocks, or a butterfly
+                               * on CGL-value rows or columns) is applied as two products
+                               * of half size, half the flops of the dense product, same
+                               * result to roundoff; 1: every operator dense (no reference
+                               * counterpart: the reference applies dense matrices) */
   double dt, gamma_imex, ra_sigma, sigma, eps_sq, beta;
 } nhqg_params;
 

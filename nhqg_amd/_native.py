@@ -35,7 +35,7 @@ _ip = C.POINTER(C.c_int32)
 class Params(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "nx", "nz", "nkx", "nky", "n_shells", "n_pairs", "substeps", "evolve_mean",
-        "advection", "w_neumann", "apply_mask", "mixed_slices")] + [
+        "advection", "w_neumann", "apply_mask", "mixed_slices", "dense_contractions")] + [
         (n, C.c_double) for n in ("dt", "gamma_imex", "ra_sigma", "sigma", "eps_sq", "beta")]
 
 
@@ -170,6 +170,7 @@ class Context:
         p.w_neumann = 1 if grid.w_bc_top == "neumann" else 0
         p.apply_mask = 1 if layout.full else 0
         p.mixed_slices = int(getattr(grid, "mixed_slices", 0))
+        p.dense_contractions = int(getattr(grid, "dense_contractions", 0))
         p.dt, p.gamma_imex = grid.dt, grid.gamma_imex
         p.ra_sigma, p.sigma = grid.Ra_sigma, grid.sigma
         p.eps_sq, p.beta = grid.mean_temp_eps_sq, grid.beta

@@ -141,6 +141,23 @@ int upload_op(nhqg_ctx* c, GemmOp* op, const double* host, int M, int K, int mt)
   return r;
 }
 
+// Reflection-split halves of an uploaded operator (gemm_split.hip), unless
+// dense contractions are asked for or the operator lacks the structure;
+// prefer_cols picks the column butterfly for operators that have both (the
+// CGL transform), so that it shares a batch with its neighbours.
+int upload_op_split(nhqg_ctx* c, GemmOp* op, const double* host, bool prefer_cols) {
+  if (!host || c->P.dense_contractions) return NHQG_OK;
+  // every operator of a context gets the tile count of the largest half
+  const int min_mth = ((c->Nq - 1) / 2 + 1 + 15) / 16;
+  std::vector<double> packed;
+  if (!gemm_split_plan(host, op->M, op->K, prefer_cols, 1e-11, min_mth, *op, packed))
+    return NHQG_OK;
+  double* d = nullptr;
+  int r = upload(c, &d, packed.data(), packed.size());
+  op->sp_At = d;
+  return r;
+}
+
 // int8 slices of an uploaded operator for the mixed-precision kernel
 int upload_op_i8(nhqg_ctx* c, GemmOp* op, const double* host, int S) {
   if (!host || S == 0 || op->K < 2) return NHQG_OK;
@@ -264,6 +281,14 @@ int gemm_jobs(nhqg_ctx* c, std::initializer_list<Job> jobs) {
   if (i8) {
     b.slices = c->mixed_slices;
     PL(CL_GEMM_I8, work, launch_gemm_i8(b, n, c->st));
+  } else if (gemm_split_batch_ok(b, n)) {
+    // the split products' own flops: 4 * (rows * depth of each half) per mode
+    double sw = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const GemmOp& o = b.job[k].op;
+      sw += 4.0 * ((double)o.sp_nr[0] * o.sp_nk[0] + (double)o.sp_nr[1] * o.sp_nk[1]) * c->nm;
+    }
+    PL(CL_GEMM, sw, launch_gemm_split(b, n, c->st));
   } else {
     PL(CL_GEMM, work, launch_gemm_batch(b, n, c->st));
   }
@@ -613,6 +638,10 @@ int check_params(const nhqg_params* p, std::string& err) {
     err = "mixed_slices must be 0 (FP64) or in [4, 7]";
     return NHQG_ERR_ARG;
   }
+  if (p->dense_contractions != 0 && p->dense_contractions != 1) {
+    err = "dense_contractions must be 0 or 1";
+    return NHQG_ERR_ARG;
+  }
   if (p->n_shells < 1) {
     err = "n_shells must be >= 1";
     return NHQG_ERR_ARG;
@@ -650,6 +679,20 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(upload_op(c, &o.W2S, h->W2S, Nq, Ng, mt));
   RUN(upload_op(c, &o.T2S, h->T2S, Nq, Ng, mt));
   RUN(upload_op(c, &o.S2T, h->S2T, Ng, Nq, mt));
+  // coefficient -> value maps split on their output rows (batched V, VSw,
+  // VSd; W2S, T2S), value -> coefficient maps on their input rows (batched
+  // Vinv, Pw, Pd; S2T)
+  RUN(upload_op_split(c, &o.V, h->V, false));
+  RUN(upload_op_split(c, &o.VSw, h->VSw, false));
+  RUN(upload_op_split(c, &o.VSd, h->VSd, false));
+  RUN(upload_op_split(c, &o.Vinv, h->Vinv, true));
+  RUN(upload_op_split(c, &o.Pw, h->Pw, true));
+  RUN(upload_op_split(c, &o.Pd, h->Pd, true));
+  RUN(upload_op_split(c, &o.Qw, h->Qw, false));
+  RUN(upload_op_split(c, &o.GSw, h->GSw, false));
+  RUN(upload_op_split(c, &o.W2S, h->W2S, false));
+  RUN(upload_op_split(c, &o.T2S, h->T2S, false));
+  RUN(upload_op_split(c, &o.S2T, h->S2T, true));
   {
     // one SBP substep cycle: Psbp = T2S * S2T (Nq x Nq)
     std::vector<double> P((size_t)Nq * Nq, 0.0);
@@ -659,6 +702,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
         for (int j = 0; j < Nq; ++j) P[(size_t)i * Nq + j] += a * h->S2T[(size_t)k * Nq + j];
       }
     RUN(upload_op(c, &o.Psbp, P.data(), Nq, Nq, mt));
+    RUN(upload_op_split(c, &o.Psbp, P.data(), false));
   }
   RUN(upload(c, &o.thb2s, h->thb2s, (size_t)Nq * Nq));
   RUN(upload(c, &o.s2thb, h->s2thb, (size_t)Nq * Nq));
@@ -736,6 +780,8 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   if (eigen) {
     RUN(upload_op(c, &o.eigQ, h->eig_Q, Ng, Ng, mt));
     RUN(upload_op(c, &o.eigQinv, h->eig_Qinv, Ng, Ng, mt));
+    RUN(upload_op_split(c, &o.eigQ, h->eig_Q, false));
+    RUN(upload_op_split(c, &o.eigQinv, h->eig_Qinv, false));
     RUN(upload_op_i8(c, &o.eigQ, h->eig_Q, c->mixed_slices));
     RUN(upload_op_i8(c, &o.eigQinv, h->eig_Qinv, c->mixed_slices));
     RUN(upload(c, &o.lam, h->eig_lam, (size_t)Ng));

@@ -162,6 +162,19 @@ struct GemmOp {
   const double* sa = nullptr;
   const double* afirst = nullptr;
   int slices = 0, nti = 0, nk = 0;
+  // Reflection split (gemm_split.hip), used when sp_At is set: two products
+  // of half size, E (half 0) and O (half 1), each at most 16*sp_mth rows.
+  // Input: ib = 0 reads rows istr*k + ioff[h] of B for half h; ib = 1 feeds
+  // E with B[k] + B[N-k] and O with B[k] - B[N-k], N = sp_nbi (B[N/2] alone
+  // to E at k = N/2).  Output: ob = 0 writes half h's row r to C row
+  // ostr*r + ooff[h] for r < nr[h]; ob = 1 writes E_r + O_r to row r and
+  // E_r - O_r to row N - r, N = sp_nbo, r <= N/2.  sp_At holds the halves
+  // k-major, sp_kp rows of 2*16*sp_mth (E rows, then O rows).
+  const double* sp_At = nullptr;
+  int sp_mth = 0, sp_kp = 0, sp_ib = 0, sp_ob = 0;
+  int sp_istr = 0, sp_ioff[2] = {0, 0}, sp_ostr = 0, sp_ooff[2] = {0, 0}, sp_nr[2] = {0, 0};
+  int sp_nk[2] = {0, 0};
+  int sp_nbi = 0, sp_nbo = 0;
 };
 // One contraction C = epilogue(A * B') of a batch.  B' = B, or with W set
 // the SBP substep update B - bc * W * g[row] with rows 0 and K-1 zeroed.
@@ -189,6 +202,17 @@ void gemm_pack_op(const double* A, int M, int K, int mt, double* out);
 // Runs b.job[0..nj) in one launch; every job has the same mt and all or
 // none use the update.  ld must be a multiple of 128.
 hipError_t launch_gemm_batch(const GemmBatch& b, int nj, hipStream_t s);
+// Reflection split of a dense M x K operator (row-major): fills the split
+// fields of op (not sp_At) and the packed halves, or returns false when the
+// operator has none of the structures within tol * max|A|.  prefer_cols
+// tries the column butterfly before the row butterfly; the halves get at
+// least min_mth row tiles (operators that share a batch share one count).
+bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol, int min_mth,
+                     GemmOp& op, std::vector<double>& packed);
+// Every job of the batch split with the same sp_mth, sp_ib and sp_ob (and
+// all with or all without the substep update).
+bool gemm_split_batch_ok(const GemmBatch& b, int nj);
+hipError_t launch_gemm_split(const GemmBatch& b, int nj, hipStream_t s);
 // Mixed precision (gemm_i8.hip): the same contract from int8 slice products,
 // b.slices slices per operand; no fused SBP update.
 bool gemm_i8_supported(int S);

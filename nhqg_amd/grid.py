@@ -303,6 +303,10 @@ class Grid:
     # slices per operand in the Chebyshev transforms and implicit-solve maps
     # (set before the first context is created)
     mixed_slices: int = 0
+    # True: apply every vertical operator as a dense product, not as the two
+    # half-size products its reflection symmetry allows (set before the
+    # first context is created)
+    dense_contractions: bool = False
     _layouts: dict = field(default_factory=dict, repr=False)
     _contexts: dict = field(default_factory=dict, repr=False)
 
@@ -564,6 +568,43 @@ class ModeLayout:
         return ModeLayout(full=full, rows=rows, nky=nky, tables=t)
 
 
+def _eigen_by_parity(B0: np.ndarray):
+    """Real eigendecomposition B0 = Q diag(lam) Q^-1 for the eigen-basis shell
+    solve.  B0 couples only Galerkin indices of equal parity (the reflection
+    z -> -z), so when that holds exactly the decomposition is taken per
+    parity block: eigenvectors of pure parity, the even block's first, and
+    Q, Q^-1 exact parity blocks.  A full eig() can mix eigenvectors of the two
+    blocks whose eigenvalues nearly coincide (at 1e-10 for Nz = 256), which
+    hides that structure.  Otherwise the full matrix is decomposed."""
+    n = B0.shape[0]
+    par = np.arange(n) % 2
+    blocks = [np.flatnonzero(par == 0), np.flatnonzero(par == 1)]
+    if np.all(B0[par[:, None] != par[None, :]] == 0.0) and blocks[1].size > 0:
+        lam = np.zeros(n)
+        Q = np.zeros((n, n))
+        Qinv = np.zeros((n, n))
+        c0 = 0
+        for ix in blocks:
+            lb, qb = np.linalg.eig(B0[np.ix_(ix, ix)])
+            _check_real_spectrum(lb, qb)
+            cs = slice(c0, c0 + ix.size)
+            lam[cs] = lb.real
+            Q[ix, cs] = np.real(qb)
+            Qinv[cs, ix] = np.linalg.inv(np.real(qb))
+            c0 += ix.size
+        return lam, Q, Qinv
+    lam, Q = np.linalg.eig(B0)
+    _check_real_spectrum(lam, Q)
+    Q = np.real(Q).copy()
+    return lam.real.copy(), Q, np.linalg.inv(Q)
+
+
+def _check_real_spectrum(lam, Q):
+    if np.abs(lam.imag).max() > 1e-12 * np.abs(lam.real).max() or np.iscomplexobj(Q) and \
+            np.abs(Q.imag).max() > 1e-12:
+        raise NotImplementedError("B0 has a complex spectrum; use shell_solver='matrices'")
+
+
 def device_operators(g: Grid, lay: ModeLayout) -> dict:
     """Fused vertical operators and physical fields for the device."""
     V, Vi, GZ = g.V, g.V_inv, g.G_Z
@@ -615,11 +656,7 @@ def device_operators(g: Grid, lay: ModeLayout) -> dict:
         # Dirichlet top: A_k = alpha_k I - c_k B0 for every shell, and B0 has
         # a real spectrum with well-conditioned eigenvectors, so the solve
         # is Q diag(1 / (alpha_k - c_k lam)) Q^-1 with no per-shell matrix
-        lam, Q = np.linalg.eig(B0)
-        if np.abs(lam.imag).max() > 1e-12 * np.abs(lam.real).max() or np.iscomplexobj(Q) and \
-                np.abs(Q.imag).max() > 1e-12:
-            raise NotImplementedError("B0 has a complex spectrum; use shell_solver='matrices'")
-        lam, Q = lam.real.copy(), np.real(Q).copy()
+        lam, Q, Qinv = _eigen_by_parity(B0)
         alpha, _, c = shell_coefficients(vals, g.cfg, g.gamma_imex, g.Ld_inv_sq, False)
         ptr, modes = lay.tables["shell_ptr"], lay.tables["shell_modes"]
         ma = np.ones(lay.nm)
@@ -628,7 +665,7 @@ def device_operators(g: Grid, lay: ModeLayout) -> dict:
             ms = modes[ptr[s]:ptr[s + 1]]
             ma[ms] = alpha[s]
             mc[ms] = c[s]
-        ops.update(shell_inv=None, eig_Q=Q, eig_Qinv=np.linalg.inv(Q), eig_lam=lam,
+        ops.update(shell_inv=None, eig_Q=Q, eig_Qinv=Qinv, eig_lam=lam,
                    mode_alpha=ma, mode_c=mc)
         return ops
     if g.shell_build == "host":
