@@ -148,7 +148,8 @@ int upload_op(nhqg_ctx* c, GemmOp* op, const double* host, int M, int K, int mt)
 int upload_op_split(nhqg_ctx* c, GemmOp* op, const double* host, bool prefer_cols) {
   if (!host || c->P.dense_contractions) return NHQG_OK;
   // every operator of a context gets the tile count of the largest half
-  const int min_mth = ((c->Nq - 1) / 2 + 1 + 15) / 16;
+  // (its Nz/2 + 1 rows: the last one on the VALU when Nz/2 is a multiple of 16)
+  const int min_mth = ((c->Nq - 1) / 2 + 15) / 16;
   std::vector<double> packed;
   if (!gemm_split_plan(host, op->M, op->K, prefer_cols, 1e-11, min_mth, *op, packed))
     return NHQG_OK;

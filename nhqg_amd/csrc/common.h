@@ -169,9 +169,10 @@ struct GemmOp {
   // to E at k = N/2).  Output: ob = 0 writes half h's row r to C row
   // ostr*r + ooff[h] for r < nr[h]; ob = 1 writes E_r + O_r to row r and
   // E_r - O_r to row N - r, N = sp_nbo, r <= N/2.  sp_At holds the halves
-  // k-major, sp_kp rows of 2*16*sp_mth (E rows, then O rows).
+  // k-major, sp_kp rows of 2*16*sp_mth + 16 (E rows, O rows, then E's row
+  // 16*sp_mth, computed on the VALU, when sp_xrow).
   const double* sp_At = nullptr;
-  int sp_mth = 0, sp_kp = 0, sp_ib = 0, sp_ob = 0;
+  int sp_mth = 0, sp_kp = 0, sp_ib = 0, sp_ob = 0, sp_xrow = 0;
   int sp_istr = 0, sp_ioff[2] = {0, 0}, sp_ostr = 0, sp_ooff[2] = {0, 0}, sp_nr[2] = {0, 0};
   int sp_nk[2] = {0, 0};
   int sp_nbi = 0, sp_nbo = 0;

@@ -27,9 +27,15 @@
 #include <algorithm>
 #include <cmath>
 
+#ifndef SP_NW
 #define SP_NW 8   // waves per workgroup
+#endif
+#ifndef SP_KC
 #define SP_KC 8   // k-chunk staged in LDS per iteration (double buffered)
+#endif
+#ifndef SP_R
 #define SP_R 2    // B ring depth in chunks
+#endif
 
 namespace {
 
@@ -126,8 +132,9 @@ gemm_split_kernel(GemmBatch bt) {
   constexpr int R = SP_R;
   constexpr int KS = KC / 4;
   constexpr int MP = MTH * 16;       // rows per half
-  constexpr int MW = 2 * MP;         // both halves: one packed row of A
-  constexpr int SA = MW + 16;        // LDS pitch: the 4 k rows of a read on distinct banks
+  constexpr int MW = 2 * MP + 16;    // one packed row of A: E, O, E's extra row (+ pad)
+  constexpr int SA = MW;             // LDS pitch, an odd multiple of 128 bytes: the 4 k
+                                     // rows of a read fall on distinct banks
   constexpr int NA = (MW * KC + NT - 1) / NT;
   constexpr bool FULL = (NA * NT == MW * KC);
   __shared__ double As[2][KC][SA];
@@ -152,6 +159,7 @@ gemm_split_kernel(GemmBatch bt) {
     acc1[t] = v4d{0.0, 0.0, 0.0, 0.0};
   }
   double ra[NA];
+  double xr = 0.0;   // E's row MP (sp_xrow): partial dot over this lane's k
   // raw B rows of the ring: (lo, hi) = (B[k], B[N-k]) for an input
   // butterfly, else the rows of half 0 and half 1
   double lo[R][KS], hi[R][KS];
@@ -242,6 +250,7 @@ gemm_split_kernel(GemmBatch bt) {
             const double a = As[cur][kr][MP + t * 16 + li];
             acc1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, vo, acc1[t], 0, 0, 0);
           }
+          xr = fma(As[cur][kr][2 * MP], ve, xr);
         }
         stashA(cur ^ 1);
         __syncthreads();
@@ -250,6 +259,19 @@ gemm_split_kernel(GemmBatch bt) {
   }
 
   split_epilogue<0, MTH, OB>(J, acc0, acc1, ld, col, lk);
+  if (J.op.sp_xrow) {
+    // E's row MP: the four k-residue partial sums sit in lanes li + 16*lk.
+    // With an output butterfly it is the middle level (O's is zero there).
+    xr += __shfl_xor(xr, 16);
+    xr += __shfl_xor(xr, 32);
+    if (lk == 0) {
+      const double v[1] = {xr};
+      const int row[1] = {OB ? MP : J.op.sp_ostr * MP + J.op.sp_ooff[0]};
+      const bool ok[1] = {true};
+      if (J.C2) epi_rows<1>(J.e2, J.C2, v, row, ok, ld, col, col >> 1);
+      epi_rows<1>(J.e, J.C, v, row, ok, ld, col, col >> 1);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -367,7 +389,9 @@ bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol
     ob = (kind == 2 || kind == 4);
     ib = (kind == 3 || kind == 4);
     if (ob) {
-      nr[0] = nr[1] = (M - 1) / 2 + 1;
+      // O's middle row is zero: the two butterfly outputs of row N/2 coincide
+      nr[0] = (M - 1) / 2 + 1;
+      nr[1] = (M - 1) / 2;
     } else {
       rs = side_parity(M, sigma);   // E: rows whose column reflection sign is +1
       nr[0] = rs.n[0];
@@ -382,12 +406,19 @@ bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol
       nk[1] = cs.n[1];
     }
   }
-  const int rmax = std::max(nr[0], nr[1]);
-  if (rmax > 16 * SP_MAXH || std::min(nr[0], nr[1]) < 1 || std::min(nk[0], nk[1]) < 1)
-    return false;
-  const int mth = std::max((rmax + 15) / 16, min_mth);
+  if (std::min(nr[0], nr[1]) < 1 || std::min(nk[0], nk[1]) < 1) return false;
+  // E's last row may go to the VALU (the dense kernel's 257th row): with
+  // nr[0] = 16*mth + 1 the halves then take mth tiles, not mth + 1.  Under an
+  // output butterfly that row must be the middle level.
+  int mth = std::max((std::max(nr[0], nr[1]) + 15) / 16, min_mth);
+  const int mx_ = std::max((std::max(nr[0] - 1, nr[1]) + 15) / 16, min_mth);
+  int xrow = 0;
+  if (nr[0] == 16 * mx_ + 1 && (!ob || 16 * mx_ == (M - 1) / 2)) {
+    mth = mx_;
+    xrow = 1;
+  }
   if (mth > SP_MAXH) return false;
-  const int MP = 16 * mth, MW = 2 * MP;
+  const int MP = 16 * mth, MW = 2 * MP + 16;
   const int kp = (std::max(nk[0], nk[1]) + SP_KC - 1) / SP_KC * SP_KC;
 
   // E/O entries
@@ -410,9 +441,14 @@ bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol
   packed.assign((size_t)kp * MW, 0.0);
   for (int h = 0; h < 2; ++h)
     for (int r = 0; r < nr[h]; ++r)
-      for (int k = 0; k < nk[h]; ++k) packed[(size_t)k * MW + h * MP + r] = entry(h, r, k);
+      for (int k = 0; k < nk[h]; ++k) {
+        // E's row MP (xrow) sits after the O rows
+        const size_t c = (h == 0 && r == MP) ? 2 * MP : (size_t)h * MP + r;
+        packed[(size_t)k * MW + c] = entry(h, r, k);
+      }
 
   op.sp_mth = mth;
+  op.sp_xrow = xrow;
   op.sp_kp = kp;
   op.sp_ib = ib;
   op.sp_ob = ob;
