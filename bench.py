@@ -37,9 +37,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_polar_256x256x256.md"
 MIXED_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mixed_step_polar_256x256x256.md")
 # kernel-class name (nhqg_profile_step) -> HIP kernels it launches
 CLASS_KERNELS = {
-    "gemm_f64_mfma": ("gemm_f64_kernel",),
+    "gemm_f64_mfma": ("gemm_split_kernel", "gemm_f64_kernel"),
     "gemm_i8_mixed": ("gemm_i8_kernel",),
-    "flux": ("col_inverse_flux", "row_flux", "col_forward_flux"),
+    "flux": ("col_inverse_flux", "row_flux2", "col_forward_flux"),
     "shell_solve": ("shell_solve_kernel",),
     "parseval_means": ("parseval_partial_kernel",),
 }
@@ -66,7 +66,9 @@ def profiled_traffic(cls: str, path: str = TRAFFIC_FILE):
 def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
     """Achieved rate of each kernel class in one profiled step, against the
     MI355X roofline that bounds it.  Work is algorithmic, in the units a
-    launch processes: 4*M*K flops per complex band mode for a contraction;
+    launch processes: for a contraction 4*(rows x depth) flops per complex
+    band mode of each product it runs (the two half-size products of a
+    reflection-split operator: half the 4*M*K of the dense product);
     16 B per complex band mode per field-level moved for the HBM-bound
     classes (inputs read once, outputs written once)."""
     nq, ng, nm = g.Nz + 1, g.Nz - 1, lay.nm
@@ -75,19 +77,21 @@ def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
     n = max(n, 1)
     ach = work / max(ms * 1e-3, 1e-30) / 1e12
     out["gemm_f64_mfma"] = dict(
-        kernel="gemm_f64_kernel (batched vertical contractions)", launches_per_step=n,
+        kernel="gemm_split_kernel (batched vertical contractions, each split into two "
+               "half-size products by the z -> -z reflection symmetry)", launches_per_step=n,
         ms_per_step=ms, avg_launch_us=1e3 * ms / n, bound="mfma", achieved=ach,
         peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=ach / PEAK_FP64_TFLOPS,
         work_per_launch=work / n,
-        work="4*M*K flops per complex mode of the 2/3 band (real operator x complex field), "
-             f"{nm} modes per contraction, 1-3 contractions per launch",
+        work="4*(rows x depth) flops per complex mode of the 2/3 band for each half-size product "
+             "(real operator x complex field; 2 products of ~129 x 129 per 257 x 257 operator, "
+             f"half the dense 4*M*K), {nm} modes per contraction, 1-3 contractions per launch",
         traffic=profiled_traffic("gemm_f64_mfma") if full_size else None)
     t = sum(prof[k][0] for k in ("flux_col_inverse", "flux_row_products", "flux_col_forward"))
     nrhs = prof["flux_col_inverse"][1]
     alg = 6.0 * 16.0 * nq * nm * nrhs          # q, w, theta nodal in + 3 tendencies out
     ach = alg / (t * 1e-3) / 1e9
     out["flux"] = dict(
-        kernel="col_inverse_flux + row_flux + col_forward_flux (one FFT-product pipeline per RHS)",
+        kernel="col_inverse_flux + row_flux2 + col_forward_flux (one FFT-product pipeline per RHS)",
         launches_per_step=3 * nrhs, ms_per_step=t, avg_launch_us=1e3 * t / (3 * nrhs), bound="hbm",
         achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
         work_per_launch=alg / (3 * nrhs),
