@@ -153,6 +153,7 @@ int upload_op_split(nhqg_ctx* c, GemmOp* op, const double* host, bool prefer_col
   std::vector<double> packed;
   if (!gemm_split_plan(host, op->M, op->K, prefer_cols, 1e-11, min_mth, *op, packed))
     return NHQG_OK;
+  op->sp_pref = prefer_cols;
   double* d = nullptr;
   int r = upload(c, &d, packed.data(), packed.size());
   op->sp_At = d;
@@ -162,6 +163,29 @@ int upload_op_split(nhqg_ctx* c, GemmOp* op, const double* host, bool prefer_col
 // int8 slices of an uploaded operator for the mixed-precision kernel
 int upload_op_i8(nhqg_ctx* c, GemmOp* op, const double* host, int S) {
   if (!host || S == 0 || op->K < 2) return NHQG_OK;
+  if (op->sp_At) {
+    // slices of the split halves (the packed FP64 halves planned again)
+    GemmOp plan = *op;
+    std::vector<double> packed;
+    const int min_mth = op->sp_mth;
+    if (gemm_split_plan(host, op->M, op->K, op->sp_pref != 0, 1e-11, min_mth, plan, packed) &&
+        plan.sp_mth == op->sp_mth) {
+      std::vector<int8_t> a8;
+      std::vector<double> sa, al;
+      int snti = 0;
+      gemm_i8_split_halves(packed, plan, S, a8, sa, al, snti);
+      int8_t* d8 = nullptr;
+      double *dsa = nullptr, *dal = nullptr;
+      int r = upload(c, &d8, a8.data(), a8.size());
+      if (!r) r = upload(c, &dsa, sa.data(), sa.size());
+      if (!r) r = upload(c, &dal, al.data(), al.size());
+      if (r) return r;
+      op->sA8 = d8;
+      op->ssa = dsa;
+      op->safirst = dal;
+      op->snti = snti;
+    }
+  }
   std::vector<int8_t> a8;
   std::vector<double> sa, al;
   gemm_i8_split(host, op->M, op->K, S, a8, sa, al);
@@ -279,7 +303,16 @@ int gemm_jobs(nhqg_ctx* c, std::initializer_list<Job> jobs) {
   // (not the SBP transfers, not the fused substep update)
   bool i8 = c->mixed_now && c->mixed_slices > 0;
   for (int k = 0; k < n && i8; ++k) i8 = b.job[k].op.A8 != nullptr && b.job[k].W == nullptr;
-  if (i8) {
+  if (i8) b.slices = c->mixed_slices;
+  if (i8 && gemm_i8_split_batch_ok(b, n)) {
+    // the split halves as int8 slice products
+    double sw = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const GemmOp& o = b.job[k].op;
+      sw += 4.0 * ((double)o.sp_nr[0] * o.sp_nk[0] + (double)o.sp_nr[1] * o.sp_nk[1]) * c->nm;
+    }
+    PL(CL_GEMM_I8, sw, launch_gemm_i8_split(b, n, c->st));
+  } else if (i8) {
     b.slices = c->mixed_slices;
     PL(CL_GEMM_I8, work, launch_gemm_i8(b, n, c->st));
   } else if (gemm_split_batch_ok(b, n)) {

@@ -63,6 +63,47 @@ NHQG_DEV double epi_apply(const Epi& e, double acc, size_t idx, int row, int mod
   return o;
 }
 
+// Epilogue of up to NV results of one lane at arbitrary output rows: the
+// arithmetic of epi_apply, with each extra field read for all NV rows at
+// once (one memory latency per term).  Rows with ok false are neither read
+// (row 0 stands in) nor stored.
+template <int NV>
+NHQG_DEV void epi_rows(const Epi& e, double* C, const double (&val)[NV], const int (&row)[NV],
+                       const bool (&ok)[NV], int ld, int col, int mode) {
+  double o[NV];
+  const double sa = e.s_acc ? e.s_acc[mode] : 1.0;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    o[i] = e.a_acc * val[i];
+    if (e.s_acc) o[i] *= sa;
+  }
+  for (int x = 0; x < e.nX; ++x) {
+    const double* X = e.X[x];
+    const double ax = e.aX[x];
+    const bool hs = e.sX[x] != nullptr;
+    const double sx = hs ? e.sX[x][mode] : 1.0;
+    double v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = X[(size_t)(ok[i] ? row[i] : 0) * ld + col];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double w = ax * v[i];
+      if (hs) w *= sx;
+      o[i] += w;
+    }
+  }
+  const double so = e.s_out ? e.s_out[mode] : 1.0;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    if (ok[i]) {
+      double w = o[i];
+      if (e.s_out) w *= so;
+      if (e.d_tab) w *= e.d_tab[(size_t)row[i] * e.d_ld + mode];
+      C[(size_t)row[i] * ld + col] = w;
+    }
+  }
+}
+
 static inline Epi epi_make(double a_acc) {
   Epi e;
   e.a_acc = a_acc;
@@ -176,6 +217,14 @@ struct GemmOp {
   int sp_istr = 0, sp_ioff[2] = {0, 0}, sp_ostr = 0, sp_ooff[2] = {0, 0}, sp_nr[2] = {0, 0};
   int sp_nk[2] = {0, 0};
   int sp_nbi = 0, sp_nbo = 0;
+  int sp_pref = 0;   // the prefer_cols the split was planned with
+  // int8 slices of the two halves (gemm_i8.hip, split kernel): tile t, half
+  // h, slice p, k-step kk at (((t*2 + h)*S + p)*2 + kk)*64 + lane (v4i);
+  // ssa / safirst at h*snti*16 + r; snti row tiles per half
+  const void* sA8 = nullptr;
+  const double* ssa = nullptr;
+  const double* safirst = nullptr;
+  int snti = 0;
 };
 // One contraction C = epilogue(A * B') of a batch.  B' = B, or with W set
 // the SBP substep update B - bc * W * g[row] with rows 0 and K-1 zeroed.
@@ -214,6 +263,14 @@ bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol
 // all with or all without the substep update).
 bool gemm_split_batch_ok(const GemmBatch& b, int nj);
 hipError_t launch_gemm_split(const GemmBatch& b, int nj, hipStream_t s);
+// int8 slices of the split halves of an operator planned by gemm_split_plan
+// (packed = its FP64 halves); fills the sA8 layout, ssa, safirst and snti
+void gemm_i8_split_halves(const std::vector<double>& packed, const GemmOp& op, int S,
+                          std::vector<int8_t>& a8, std::vector<double>& sa,
+                          std::vector<double>& afirst, int& snti);
+// every job carries split slices with the same sp_ib / sp_ob, no substep update
+bool gemm_i8_split_batch_ok(const GemmBatch& b, int nj);
+hipError_t launch_gemm_i8_split(const GemmBatch& b, int nj, hipStream_t s);
 // Mixed precision (gemm_i8.hip): the same contract from int8 slice products,
 // b.slices slices per operand; no fused SBP update.
 bool gemm_i8_supported(int S);

@@ -41,47 +41,6 @@ namespace {
 
 constexpr int SP_MAXH = 9;   // row tiles per half: M <= 2 * 144
 
-// Epilogue of up to NV results of one lane at arbitrary output rows: the
-// arithmetic of epi_apply, with each extra field read for all NV rows at
-// once (one memory latency per term).  Rows with ok false are neither read
-// (row 0 stands in) nor stored.
-template <int NV>
-NHQG_DEV void epi_rows(const Epi& e, double* C, const double (&val)[NV], const int (&row)[NV],
-                       const bool (&ok)[NV], int ld, int col, int mode) {
-  double o[NV];
-  const double sa = e.s_acc ? e.s_acc[mode] : 1.0;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    o[i] = e.a_acc * val[i];
-    if (e.s_acc) o[i] *= sa;
-  }
-  for (int x = 0; x < e.nX; ++x) {
-    const double* X = e.X[x];
-    const double ax = e.aX[x];
-    const bool hs = e.sX[x] != nullptr;
-    const double sx = hs ? e.sX[x][mode] : 1.0;
-    double v[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = X[(size_t)(ok[i] ? row[i] : 0) * ld + col];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double w = ax * v[i];
-      if (hs) w *= sx;
-      o[i] += w;
-    }
-  }
-  const double so = e.s_out ? e.s_out[mode] : 1.0;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    if (ok[i]) {
-      double w = o[i];
-      if (e.s_out) w *= so;
-      if (e.d_tab) w *= e.d_tab[(size_t)row[i] * e.d_ld + mode];
-      C[(size_t)row[i] * ld + col] = w;
-    }
-  }
-}
-
 }  // namespace
 
 // Epilogue of tile T and up.  v_mfma_f64_16x16x4_f64 output: D[i][j] of a
