@@ -38,7 +38,7 @@ MIXED_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mixed_step_polar_256x256x25
 # kernel-class name (nhqg_profile_step) -> HIP kernels it launches
 CLASS_KERNELS = {
     "gemm_f64_mfma": ("gemm_split_kernel", "gemm_f64_kernel"),
-    "gemm_i8_mixed": ("gemm_i8_kernel",),
+    "gemm_i8_mixed": ("gemm_i8_split_kernel", "gemm_i8_kernel"),
     "flux": ("col_inverse_flux", "row_flux2", "col_forward_flux"),
     "shell_solve": ("shell_solve_kernel",),
     "parseval_means": ("parseval_partial_kernel",),
@@ -104,12 +104,14 @@ def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
         ach = work / (ms * 1e-3) / 1e12
         ops = npair * work / (ms * 1e-3) / 1e12
         out["gemm_i8_mixed"] = dict(
-            kernel=f"gemm_i8_kernel<{slices}> (int8 slice contractions, FP64 reconstruction)",
+            kernel=f"gemm_i8_split_kernel<{slices}> (int8 slice products of the reflection-split "
+                   "halves, FP64 reconstruction)",
             launches_per_step=n, ms_per_step=ms, avg_launch_us=1e3 * ms / n, bound="mfma",
             achieved=ops, peak=PEAK_INT8_TOPS, unit="TOP/s", frac=ops / PEAK_INT8_TOPS,
             fp64_equivalent_tflops=ach, work_per_launch=work / n,
-            work=f"4*M*K flops per complex band mode, executed as {npair} int8 slice products "
-                 "(achieved counts int8 ops on the unpadded M x K; tile padding not counted)",
+            work=f"4*(rows x depth) flops per complex band mode of each half-size product, "
+                 f"executed as {npair} int8 slice products (achieved counts int8 ops on the "
+                 "unpadded halves; tile padding not counted)",
             traffic=profiled_traffic("gemm_i8_mixed", MIXED_TRAFFIC_FILE)
             if full_size else None)
     ms, n, work = prof["shell_solve"]

@@ -172,8 +172,8 @@ int upload_op_i8(nhqg_ctx* c, GemmOp* op, const double* host, int S) {
         plan.sp_mth == op->sp_mth) {
       std::vector<int8_t> a8;
       std::vector<double> sa, al;
-      int snti = 0;
-      gemm_i8_split_halves(packed, plan, S, a8, sa, al, snti);
+      int snti = 0, snkh = 0;
+      gemm_i8_split_halves(packed, plan, S, a8, sa, al, snti, snkh);
       int8_t* d8 = nullptr;
       double *dsa = nullptr, *dal = nullptr;
       int r = upload(c, &d8, a8.data(), a8.size());
@@ -184,6 +184,7 @@ int upload_op_i8(nhqg_ctx* c, GemmOp* op, const double* host, int S) {
       op->ssa = dsa;
       op->safirst = dal;
       op->snti = snti;
+      op->snkh = snkh;
     }
   }
   std::vector<int8_t> a8;

@@ -219,12 +219,13 @@ struct GemmOp {
   int sp_nbi = 0, sp_nbo = 0;
   int sp_pref = 0;   // the prefer_cols the split was planned with
   // int8 slices of the two halves (gemm_i8.hip, split kernel): tile t, half
-  // h, slice p, k-step kk at (((t*2 + h)*S + p)*2 + kk)*64 + lane (v4i);
-  // ssa / safirst at h*snti*16 + r; snti row tiles per half
+  // h, slice p, k-step kk at (((t*2 + h)*S + p)*snkh + kk)*64 + lane (v4i);
+  // ssa / safirst at h*snti*16 + r; snti row tiles per half, snkh k-steps
+  // of 64 per half (1 or 2)
   const void* sA8 = nullptr;
   const double* ssa = nullptr;
   const double* safirst = nullptr;
-  int snti = 0;
+  int snti = 0, snkh = 0;
 };
 // One contraction C = epilogue(A * B') of a batch.  B' = B, or with W set
 // the SBP substep update B - bc * W * g[row] with rows 0 and K-1 zeroed.
@@ -264,10 +265,10 @@ bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol
 bool gemm_split_batch_ok(const GemmBatch& b, int nj);
 hipError_t launch_gemm_split(const GemmBatch& b, int nj, hipStream_t s);
 // int8 slices of the split halves of an operator planned by gemm_split_plan
-// (packed = its FP64 halves); fills the sA8 layout, ssa, safirst and snti
+// (packed = its FP64 halves); fills the sA8 layout, ssa, safirst, snti, snkh
 void gemm_i8_split_halves(const std::vector<double>& packed, const GemmOp& op, int S,
                           std::vector<int8_t>& a8, std::vector<double>& sa,
-                          std::vector<double>& afirst, int& snti);
+                          std::vector<double>& afirst, int& snti, int& snkh);
 // every job carries split slices with the same sp_ib / sp_ob, no substep update
 bool gemm_i8_split_batch_ok(const GemmBatch& b, int nj);
 hipError_t launch_gemm_i8_split(const GemmBatch& b, int nj, hipStream_t s);

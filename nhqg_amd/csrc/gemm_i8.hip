@@ -279,11 +279,11 @@ void gemm_i8_split(const double* A, int M, int K, int S, std::vector<int8_t>& a8
 // follow the FP64 split kernel.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int GI_NKH = 2;       // k-steps of 64 per half
 constexpr int GI_MAXT = 9;      // row tiles per half
 }  // namespace
 
-template <int S, bool IB, bool OB>
+// GI_NKH: k-steps of 64 per half (1 for halves at most 65 deep, else 2)
+template <int S, int GI_NKH, bool IB, bool OB>
 __global__ void __launch_bounds__(GI_NW * 64)
 gemm_i8_split_kernel(GemmBatch bt) {
   constexpr int NT = GI_NW * 64;
@@ -483,34 +483,45 @@ bool gemm_i8_split_batch_ok(const GemmBatch& b, int nj) {
     const GemmJob& J = b.job[j];
     const GemmOp& p = J.op;
     if (!p.sA8 || !p.ssa || !p.safirst || p.slices != b.slices || p.sp_ib != o.sp_ib ||
-        p.sp_ob != o.sp_ob || p.snti < 1 || p.snti > GI_MAXT ||
-        std::max(p.sp_nk[0], p.sp_nk[1]) > GI_NKH * 64 + 1 || J.W || !J.B || !J.C)
+        p.sp_ob != o.sp_ob || p.snkh != o.snkh || p.snkh < 1 || p.snkh > 2 || p.snti < 1 ||
+        p.snti > GI_MAXT || std::max(p.sp_nk[0], p.sp_nk[1]) > p.snkh * 64 + 1 || J.W ||
+        !J.B || !J.C)
       return false;
   }
   return true;
 }
 
-template <int S>
+template <int S, int NKH>
 static void launch_i8s(const GemmBatch& b, int nj, hipStream_t s) {
   const GemmOp& o = b.job[0].op;
   dim3 grid(b.ld / (GI_NW * 16), nj);
   if (o.sp_ib && o.sp_ob)
-    hipLaunchKernelGGL((gemm_i8_split_kernel<S, true, true>), grid, dim3(GI_NW * 64), 0, s, b);
+    hipLaunchKernelGGL((gemm_i8_split_kernel<S, NKH, true, true>), grid, dim3(GI_NW * 64), 0, s,
+                       b);
   else if (o.sp_ib)
-    hipLaunchKernelGGL((gemm_i8_split_kernel<S, true, false>), grid, dim3(GI_NW * 64), 0, s, b);
+    hipLaunchKernelGGL((gemm_i8_split_kernel<S, NKH, true, false>), grid, dim3(GI_NW * 64), 0, s,
+                       b);
   else if (o.sp_ob)
-    hipLaunchKernelGGL((gemm_i8_split_kernel<S, false, true>), grid, dim3(GI_NW * 64), 0, s, b);
+    hipLaunchKernelGGL((gemm_i8_split_kernel<S, NKH, false, true>), grid, dim3(GI_NW * 64), 0, s,
+                       b);
   else
-    hipLaunchKernelGGL((gemm_i8_split_kernel<S, false, false>), grid, dim3(GI_NW * 64), 0, s, b);
+    hipLaunchKernelGGL((gemm_i8_split_kernel<S, NKH, false, false>), grid, dim3(GI_NW * 64), 0,
+                       s, b);
+}
+
+template <int S>
+static void launch_i8s_k(const GemmBatch& b, int nj, hipStream_t s) {
+  if (b.job[0].op.snkh == 1) launch_i8s<S, 1>(b, nj, s);
+  else launch_i8s<S, 2>(b, nj, s);
 }
 
 hipError_t launch_gemm_i8_split(const GemmBatch& b, int nj, hipStream_t s) {
   if (!gemm_i8_split_batch_ok(b, nj)) return hipErrorInvalidValue;
   switch (b.slices) {
-    case 4: launch_i8s<4>(b, nj, s); break;
-    case 5: launch_i8s<5>(b, nj, s); break;
-    case 6: launch_i8s<6>(b, nj, s); break;
-    default: launch_i8s<7>(b, nj, s); break;
+    case 4: launch_i8s_k<4>(b, nj, s); break;
+    case 5: launch_i8s_k<5>(b, nj, s); break;
+    case 6: launch_i8s_k<6>(b, nj, s); break;
+    default: launch_i8s_k<7>(b, nj, s); break;
   }
   return hipGetLastError();
 }
@@ -521,9 +532,11 @@ hipError_t launch_gemm_i8_split(const GemmBatch& b, int nj, hipStream_t s) {
 // FP64, k >= 1 in S slices with one exponent per half row.
 void gemm_i8_split_halves(const std::vector<double>& packed, const GemmOp& op, int S,
                           std::vector<int8_t>& a8, std::vector<double>& sa,
-                          std::vector<double>& afirst, int& snti) {
+                          std::vector<double>& afirst, int& snti, int& snkh) {
   const int MP = 16 * op.sp_mth, MW = 2 * MP + 16;
   snti = (std::max(op.sp_nr[0], op.sp_nr[1]) + 15) / 16;
+  snkh = std::max(1, (std::max(op.sp_nk[0], op.sp_nk[1]) - 1 + 63) / 64);
+  const int GI_NKH = snkh;
   a8.assign((size_t)snti * 2 * S * GI_NKH * 64 * 16, 0);
   sa.assign((size_t)2 * snti * 16, 0.0);
   afirst.assign((size_t)2 * snti * 16, 0.0);
