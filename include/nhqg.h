@@ -183,6 +183,9 @@ int nhqg_diagnostics(nhqg_ctx *ctx, double *out, int32_t n_out);
  *   NHQG_OP_CORRECTOR       state <- SBP corrector(state, total_dt=arg)
  *                                                         solver.py:909-950
  *   NHQG_OP_SANITIZE        state <- finalize + sanitize  solver.py:128-146
+ *   NHQG_OP_RK4_STEP        state <- rk4_step(state), the explicit RK4
+ *                           validator (fixed_conduction configs; evolve_mean
+ *                           returns NHQG_ERR_UNSUPPORTED)  solver.py:1276-1336
  * nhqg_get_aux copies the aux fields (same layout as the state, th_bar
  * part returned as zeros). */
 #define NHQG_OP_EXPLICIT_RHS 1
@@ -190,6 +193,7 @@ int nhqg_diagnostics(nhqg_ctx *ctx, double *out, int32_t n_out);
 #define NHQG_OP_IMPLICIT_SOLVE 3
 #define NHQG_OP_CORRECTOR 4
 #define NHQG_OP_SANITIZE 5
+#define NHQG_OP_RK4_STEP 6
 int nhqg_apply(nhqg_ctx *ctx, int32_t op, double arg);
 int nhqg_get_aux(nhqg_ctx *ctx, double *q, double *w, double *th);
 

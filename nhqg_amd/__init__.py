@@ -6,10 +6,11 @@ with the same ``NHQGConfig`` fields and ``State`` layout.
 
 from nhqg_amd.config import NHQGConfig
 from nhqg_amd.grid import Grid, make_grid
-from nhqg_amd.solver import State, Stepper, imex_step, is_band_limited, run, sanitize_state
+from nhqg_amd.solver import (State, Stepper, imex_step, is_band_limited, rk4_step, run,
+                             sanitize_state)
 from nhqg_amd.diagnostics import compute_diagnostics
 
 __all__ = [
     "NHQGConfig", "Grid", "make_grid", "State", "Stepper", "imex_step", "is_band_limited",
-    "run", "sanitize_state", "compute_diagnostics",
+    "rk4_step", "run", "sanitize_state", "compute_diagnostics",
 ]

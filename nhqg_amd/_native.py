@@ -22,6 +22,7 @@ OP_IMPLICIT_TEND = 2
 OP_IMPLICIT_SOLVE = 3
 OP_CORRECTOR = 4
 OP_SANITIZE = 5
+OP_RK4_STEP = 6
 MLP_ODD = 1
 DIAG_NAMES = ("KE_tot", "KE_bt", "KE_bc", "max_speed", "enstrophy", "Nusselt",
               "Nusselt_dealiased", "vol_avg_tw_dealiased", "th_bar_max", "q_rms", "w_rms",

@@ -57,6 +57,20 @@ def imex_step(state: State, grid: Grid) -> State:
     return State(*ctx.get_state())
 
 
+def rk4_step(state: State, grid: Grid) -> State:
+    """One explicit RK4 step of the full RHS (explicit + implicit tendency),
+    the reference's validator (nhqg/solver.py:1276-1336), on the GPU: stage
+    combinations, the RK4-only post-step dissipation exp(-rate dt), th_bar's
+    Dirichlet projection and sanitation.  thermal_closure='fixed_conduction'
+    configs only: the evolve_mean RHS with the mean-temperature coupling is
+    not built (NativeError, NHQG_ERR_UNSUPPORTED)."""
+    st = _as_state(state)
+    ctx = grid.context(full=not is_band_limited(st, grid))
+    ctx.set_state(*st)
+    ctx.apply(_native.OP_RK4_STEP)
+    return State(*ctx.get_state())
+
+
 def sanitize_state(state: State, grid: Grid) -> State:
     """Hermitian projection of the self-conjugate rfft2 columns and 2/3 band
     limit (nhqg/solver.py:107-146), on the host."""

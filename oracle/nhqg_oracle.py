@@ -583,6 +583,46 @@ def imex_step(s: OState, o) -> OState:
     return sanitize(new, o)
 
 
+def full_rhs(s: OState, o) -> OState:
+    """explicit + implicit tendency (solver.py:1249-1258); under
+    fixed_conduction the full-grid explicit RHS is the base-grid one."""
+    E = explicit_rhs(s, o)
+    I = implicit_tendency(s, o)
+    return OState(E.q + I.q, E.w + I.w, E.th + I.th, E.thb + I.thb)
+
+
+def apply_bcs(s: OState, o) -> OState:
+    """_apply_bcs (solver.py:1261-1273), q_boundary='none'."""
+    w = vmat(o.w_pinv, vmat(o.proj_w, vmat(o.w_stencil, s.w)))
+    th = vmat(o.d_pinv, vmat(o.proj_d, vmat(o.S_d, s.th)))
+    return vertical_cutoff(OState(s.q, w, th, o.proj_d @ s.thb), o)
+
+
+def rk4_step(s: OState, o) -> OState:
+    """rk4_step, the explicit validator (solver.py:1276-1336)."""
+    if o.thermal_closure != "fixed_conduction":
+        raise NotImplementedError("oracle rk4_step covers fixed_conduction")
+    dt = o.dt
+
+    def axpy(a, x, k):
+        return OState(x.q + a * k.q, x.w + a * k.w, x.th + a * k.th, x.thb + a * k.thb)
+
+    k1 = full_rhs(s, o)
+    k2 = full_rhs(apply_bcs(axpy(0.5 * dt, s, k1), o), o)
+    k3 = full_rhs(apply_bcs(axpy(0.5 * dt, s, k2), o), o)
+    k4 = full_rhs(apply_bcs(axpy(dt, s, k3), o), o)
+    comb = [getattr(s, f) + (dt / 6) * (getattr(k1, f) + 2 * getattr(k2, f) + 2 * getattr(k3, f)
+                                         + getattr(k4, f)) for f in ("q", "w", "th", "thb")]
+    # post-step dissipation exp(-rate dt), RK4 only (grid.py:682-684)
+    q = comb[0] * np.exp(-o.diss_q * dt)[None]
+    w = comb[1] * np.exp(-o.diss_w * dt)[None]
+    th = comb[2] * np.exp(-o.diss_th * dt)[None]
+    for f in (q, w, th):
+        f[:, 0, 0] = 0.0                                    # zero_mean
+    new = apply_bcs(OState(q, w, th, comb[3]), o)
+    return sanitize(new, o)
+
+
 # ---------------------------------------------------------------------------
 # Initial states
 # ---------------------------------------------------------------------------
