@@ -79,6 +79,44 @@ def add_barotropic_flow(state: State, grid: Grid, seed: int = 1, u_rms: float = 
     return sanitize_state(State(q, state.w_hat, state.th_hat, state.th_bar), grid)
 
 
+def barotropic_vorticity_state(grid: Grid, seed: int, k_peak: float, k_width: float,
+                                amplitude: float, confine_radius: float | None = None) -> State:
+    """The driver's annular barotropic vorticity initial state
+    (scripts/run_polar.py:115-158): white noise filtered by a Gaussian
+    annulus |k| ~ k_peak (width k_width, k = 0 removed) and the 2/3 band,
+    optionally tapered to a polar cap of radius confine_radius around the
+    domain centre (tanh ramp two initial wavelengths wide, band-limited
+    again), scaled to standard deviation `amplitude` (over the cap when
+    confined), placed in the T_0 (barotropic) coefficient of q; w, theta and
+    th_bar zero."""
+    Nx, Nz = grid.Nx, grid.Nz
+    L = grid.cfg.L
+    rng = np.random.default_rng(seed)
+    zhat = np.fft.rfft2(rng.standard_normal((Nx, Nx)))
+    kx = 2.0 * np.pi * np.fft.fftfreq(Nx, d=L / Nx)[:, None]
+    ky = 2.0 * np.pi * np.arange(Nx // 2 + 1)[None, :] / L
+    ann = np.exp(-0.5 * ((np.sqrt(kx ** 2 + ky ** 2) - k_peak) / k_width) ** 2)
+    ann[0, 0] = 0.0
+    kxi = np.rint(np.fft.fftfreq(Nx, d=1.0 / Nx)).astype(int)[:, None]
+    kyi = np.arange(Nx // 2 + 1)[None, :]
+    band = (np.abs(kxi) <= Nx // 3) & (kyi <= Nx // 3)
+    zeta = np.fft.irfft2(zhat * ann * band, s=(Nx, Nx))
+    if confine_radius is not None:
+        xg = np.arange(Nx) * L / Nx
+        X, Y = np.meshgrid(xg, xg, indexing="ij")
+        r = np.hypot(X - L / 2.0, Y - L / 2.0)
+        ramp = 2.0 * 2.0 * np.pi / k_peak
+        zeta = zeta * 0.5 * (1.0 - np.tanh((r - confine_radius) / ramp))
+        zeta = np.fft.irfft2(np.fft.rfft2(zeta) * band, s=(Nx, Nx))
+        zeta *= amplitude / zeta[r < confine_radius].std()
+    else:
+        zeta *= amplitude / zeta.std()
+    q = np.zeros((Nz + 1, Nx, Nx // 2 + 1), dtype=np.complex128)
+    q[0] = np.fft.rfft2(zeta)
+    zg = np.zeros((Nz - 1, Nx, Nx // 2 + 1), dtype=np.complex128)
+    return State(q, zg, zg.copy(), np.zeros(Nz + 1))
+
+
 def polar_config_256(**kw):
     """Plain FP64 production step on one MI355X: Nx=Ny=256, Nz=256,
     Ra=100, Pr=1, box 24 onset wavelengths, polar trap gamma=2.2e-3

@@ -8,8 +8,8 @@ import numpy as np
 import pytest
 
 from nhqg_amd import make_grid
-from nhqg_amd.init import (inject_cyclones, thermal_noise_state, turbulent_state,
-                           upsample_spectral, upsample_state)
+from nhqg_amd.init import (barotropic_vorticity_state, inject_cyclones, thermal_noise_state,
+                           turbulent_state, upsample_spectral, upsample_state)
 from tests import cases
 
 
@@ -70,3 +70,35 @@ def test_inject_cyclones_touches_only_the_barotropic_level():
     # two Gaussians of peak 5 (slightly smoothed by the 2/3 band)
     assert 4.5 < d.max() < 10.5
     assert np.abs((out.q_hat[0] - s.q_hat[0]) * (1.0 - g.mask_23)).max() == 0.0
+
+
+@pytest.mark.parametrize("confine", [None, 6.0])
+def test_barotropic_vorticity_state(confine):
+    """scripts/run_polar.py:115-158 by its properties: vorticity standard
+    deviation = amplitude (over the cap when confined), only the barotropic
+    T_0 level set, band-limited, no mean and the annulus's spectrum peaked
+    near k_peak (unconfined), deterministic in the seed."""
+    g = make_grid(cases.fixture_cfg(Nx=64, L=20.0))
+    kp, kw, amp = 2.0, 0.5, 3.0
+    s = barotropic_vorticity_state(g, 5, kp, kw, amp, confine)
+    zeta = np.fft.irfft2(s.q_hat[0], s=(64, 64))
+    if confine is None:
+        assert abs(zeta.std() - amp) < 1e-12 * amp
+        spec = np.abs(s.q_hat[0]) ** 2
+        k = np.sqrt(g.ksq)
+        kbar = (spec * k).sum() / spec.sum()
+        assert abs(kbar - kp) < 2 * kw
+    else:
+        xg = np.arange(64) * 20.0 / 64
+        X, Y = np.meshgrid(xg, xg, indexing="ij")
+        inside = np.hypot(X - 10.0, Y - 10.0) < confine
+        assert abs(zeta[inside].std() - amp) < 1e-12 * amp
+        assert np.abs(zeta[~inside]).max() < np.abs(zeta[inside]).max()
+    assert np.all(s.q_hat[1:] == 0) and np.all(s.w_hat == 0) and np.all(s.th_hat == 0)
+    assert np.all(s.th_bar == 0)
+    assert np.abs(s.q_hat[0] * (1.0 - g.mask_23)).max() < 1e-12 * np.abs(s.q_hat[0]).max()
+    if confine is None:
+        # (the cap taper leaves a mean; the solver's k = 0 projection removes it)
+        assert abs(s.q_hat[0, 0, 0]) < 1e-12 * np.abs(s.q_hat[0]).max()
+    s2 = barotropic_vorticity_state(g, 5, kp, kw, amp, confine)
+    assert np.array_equal(s2.q_hat, s.q_hat)
