@@ -28,7 +28,7 @@
 #include <cmath>
 
 #ifndef SP_NW
-#define SP_NW 8   // waves per workgroup
+#define SP_NW 4   // waves per workgroup (two workgroups per CU)
 #endif
 #ifndef SP_KC
 #define SP_KC 8   // k-chunk staged in LDS per iteration (double buffered)
