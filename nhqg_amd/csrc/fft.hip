@@ -23,7 +23,8 @@
 // A "team" of Nx/4 threads owns one transform.  Row passes run 256-thread
 // blocks (1024/Nx teams); the column passes run ColBlock<N> threads, so a
 // block holds at least 4 columns and their transposed stores and gathers
-// move >= 64-byte runs (4 columns at Nx = 1024, 8 at 512).  The trap background is added to q' in physical space.
+// move >= 64-byte runs (4 columns at Nx = 1024, 8 at 512 and 256).  The
+// trap background is added to q' in physical space.
 // The sponge rides in the same passes but, as in the reference, acts on
 // the coefficient levels of q', w and theta (X5 slots 5-7, Y6 slots 6-8):
 // acting on nodal values instead is equal in exact arithmetic, but the
@@ -40,9 +41,13 @@ template <> struct Log2<256> { static constexpr int v = 8; };
 template <> struct Log2<512> { static constexpr int v = 9; };
 template <> struct Log2<1024> { static constexpr int v = 10; };
 
+// Threads per column-pass workgroup: 8 columns (128-byte runs of every
+// X5 / Y6 row) at N = 256 and 512, 4 at N = 1024, N/4 teams of N/4 threads
+// below (at N = 256, 4-column blocks wrote 21% more than the algorithmic
+// bytes and made the inverse pass 0.67 instead of 0.55 ms per step).
 template <int N>
 struct ColBlock {
-  static constexpr int v = (N >= 512) ? 1024 : 256;
+  static constexpr int v = (N >= 512) ? 1024 : (N == 256 ? 512 : 256);
 };
 
 // LDS layout of a team's transform buffer.  Element i of a team's N-point
