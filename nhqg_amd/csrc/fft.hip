@@ -42,9 +42,10 @@ template <> struct Log2<512> { static constexpr int v = 9; };
 template <> struct Log2<1024> { static constexpr int v = 10; };
 
 // Threads per column-pass workgroup: 8 columns (128-byte runs of every
-// X5 / Y6 row) at N = 256 and 512, 4 at N = 1024, N/4 teams of N/4 threads
-// below (at N = 256, 4-column blocks wrote 21% more than the algorithmic
-// bytes and made the inverse pass 0.67 instead of 0.55 ms per step).
+// X5 / Y6 row) at N = 256 and 512, 4 at N = 1024, and 256 threads (1024/N
+// columns) below.  At N = 256, 4-column blocks wrote 21% more than the
+// algorithmic bytes and made the inverse pass 0.67 instead of 0.55 ms per
+// step.
 template <int N>
 struct ColBlock {
   static constexpr int v = (N >= 512) ? 1024 : (N == 256 ? 512 : 256);
