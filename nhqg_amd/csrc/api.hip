@@ -348,8 +348,13 @@ int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
   } while (0)
 
 // explicit_rhs on the base grid (nhqg/solver.py:510-599; E_th_bar = 0).
+// With R set, the back-transform contractions also write R_f = y_f + al*E_f
+// (the stage-1 predictor right-hand sides) from the same accumulators, for
+// each field whose E is final there (E_q only without beta and closure);
+// the return value flags the fields done that way (bit f).
 int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, double* Eq,
-                 double* Ew, double* Eth) {
+                 double* Ew, double* Eth, double* const* R = nullptr, double al = 0.0,
+                 int* fused = nullptr) {
   const Ops& o = c->o;
   RUN(gemm_jobs(c, {Job{&o.V, q, c->qn, epi_make(1.0)}, Job{&o.VSw, w, c->wn, epi_make(1.0)},
                     Job{&o.VSd, th, c->thn, epi_make(1.0)}}));
@@ -402,8 +407,25 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
     epi_add(ew, -1.0, c->spw);
     epi_add(et, -1.0, c->spth);
   }
-  RUN(gemm_jobs(c, {Job{&o.Vinv, c->qn, Eq, eq}, Job{&o.Pw, c->wn, Ew, ew},
-                    Job{&o.Pd, c->thn, Eth, et}}));
+  Job jq{&o.Vinv, c->qn, Eq, eq}, jw{&o.Pw, c->wn, Ew, ew}, jt{&o.Pd, c->thn, Eth, et};
+  if (fused) *fused = 0;
+  if (R) {
+    // R = y + al * E = y - al * acc - al * sponge
+    Job* js[3] = {&jq, &jw, &jt};
+    const double* ys[3] = {q, w, th};
+    double* sps[3] = {c->spq, c->spw, c->spth};
+    const bool q_final = c->P.beta == 0.0 && !c->cl_on;
+    for (int f = 0; f < 3; ++f) {
+      if (f == 0 && !q_final) continue;
+      Epi e2 = epi_make(-al);
+      epi_add(e2, 1.0, ys[f]);
+      if (a.sponge_phys) epi_add(e2, -al, sps[f]);
+      js[f]->C2 = R[f];
+      js[f]->e2 = e2;
+      if (fused) *fused |= 1 << f;
+    }
+  }
+  RUN(gemm_jobs(c, {jq, jw, jt}));
   if (c->P.beta != 0.0)
     LK(launch_beta(c->Nq, c->ldc, c->nm, c->P.beta, o.kx_mode, o.inv_denom, (const double2*)q,
                    (double2*)Eq, c->st));
@@ -545,21 +567,22 @@ int enqueue_step_body(nhqg_ctx* c) {
   const int Nq = c->Nq, Ng = c->Ng;
   const bool em = c->P.evolve_mean != 0;
 
-  // stage 1 predictor
-  RUN(explicit_rhs(c, c->yq, c->yw, c->yth, c->E1q, c->E1w, c->E1th));
+  // stage 1 predictor; R = y + al*E1 comes out of the back transforms
+  // where E1 is final there
+  double* const R1[3] = {c->Rq, c->Rw, c->Rth};
+  int fused = 0;
+  RUN(explicit_rhs(c, c->yq, c->yw, c->yth, c->E1q, c->E1w, c->E1th, R1, al, &fused));
   {
-    Epi e = epi_make(0.0);
-    epi_add(e, 1.0, c->yq);
-    epi_add(e, al, c->E1q);
-    RUN(combine(c, Nq, c->Rq, e));
-    Epi f = epi_make(0.0);
-    epi_add(f, 1.0, c->yw);
-    epi_add(f, al, c->E1w);
-    RUN(combine(c, Ng, c->Rw, f));
-    Epi h = epi_make(0.0);
-    epi_add(h, 1.0, c->yth);
-    epi_add(h, al, c->E1th);
-    RUN(combine(c, Ng, c->Rth, h));
+    const double* ys[3] = {c->yq, c->yw, c->yth};
+    const double* es[3] = {c->E1q, c->E1w, c->E1th};
+    const int nl[3] = {Nq, Ng, Ng};
+    for (int f = 0; f < 3; ++f) {
+      if (fused & (1 << f)) continue;
+      Epi e = epi_make(0.0);
+      epi_add(e, 1.0, ys[f]);
+      epi_add(e, al, es[f]);
+      RUN(combine(c, nl[f], R1[f], e));
+    }
   }
   RUN(implicit_solve(c, c->Rq, c->Rw, c->Rth, c->s1q, c->s1w, c->s1th));
   PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->ythb, c->s1thb, c->st));
