@@ -354,7 +354,32 @@ NHQG_DEV void fwd_pair(double2* ba, double2* bb, int t, const double2* tw, const
                        const double* b, int nky, double2* A, double2* B) {
   constexpr int T = N / 4;
   const double2* r;
-  if constexpr (Log2<N>::v % 2 == 0) {
+  if constexpr (Log2<N>::v % 2 == 0 && T <= 64) {
+    // The team is one wavefront (or part of one) and the bins come back in
+    // registers: bin t + s*T of this lane pairs with bin N - t - s*T =
+    // (3-s)*T + (T-t), register 3-s of lane T-t (for t = 0, register
+    // (4-s) % 4 of this lane), fetched by cross-lane shuffles, no LDS.
+    double2 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = make_double2(a[j], b ? b[j] : 0.0);
+    team_fft_reg<N, false>(v, ba, bb, t, tw);
+    const int lane = threadIdx.x & 63;
+    const int src = (lane & ~(T - 1)) | ((T - t) & (T - 1));
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      double2 zm = make_double2(__shfl(v[3 - s].x, src, 64), __shfl(v[3 - s].y, src, 64));
+      if (t == 0) zm = v[(4 - s) & 3];
+      const double2 zk = v[s];
+      if (t + s * T < nky) {
+        A[s] = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+        B[s] = make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
+      } else {
+        A[s] = make_double2(0.0, 0.0);
+        B[s] = make_double2(0.0, 0.0);
+      }
+    }
+    return;
+  } else if constexpr (Log2<N>::v % 2 == 0) {
     // the bins of the thread come back in registers; the pairing with bin
     // N - ky needs every bin of the team, through LDS
     double2 v[4];
@@ -579,7 +604,7 @@ row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, doubl
 // gradient (J(psi, q + eta)).
 // ---------------------------------------------------------------------------
 template <int N, bool SPONGE, bool TRAP>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 row_jac(int nky, int ng, const double2* __restrict__ X, size_t fstride, double2* __restrict__ Y,
         const double* __restrict__ ky_col, const double* __restrict__ eta_x,
         const double* __restrict__ eta_y, const double* __restrict__ sponge_phys,
