@@ -496,9 +496,9 @@ NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, 
 // 44 VGPRs and drops occupancy from 3 to 2 waves per SIMD, which loses more
 // than the hidden load latency gains.  The pass is bound by LDS bandwidth;
 // its transforms start and end in registers (team_fft_reg) where log2 N is
-// even.  That needs a few more VGPRs than 3 waves per SIMD allow, and the
-// kernel is held at 3 (amdgpu_waves_per_eu) at the cost of a few spilled
-// registers in the trap variant: 15% faster than 2 waves without spills.
+// even, and the forward transforms pair bins N - k across lanes by shuffles.
+// The kernel is held at 3 waves per SIMD (amdgpu_waves_per_eu): 15% faster
+// than 2 waves (at N = 1024 it spills a few registers to stay there).
 // ---------------------------------------------------------------------------
 template <int N, bool SPONGE, bool TRAP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
