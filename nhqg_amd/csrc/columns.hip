@@ -322,7 +322,7 @@ hipError_t launch_parseval(int nl, int nm, int ld, int P, const double* a, const
 // Laplacian as [i*3 + (j-i+1)].
 // ---------------------------------------------------------------------------
 #define SBP_NMAX 272
-#define SBP_MV_ROWS 8
+#define SBP_MV_ROWS 2   // rows per wavefront of small_matvec_kernel
 
 // y[i] = sum_j M[i][j] x[j] for a small dense row-major M, one wavefront per
 // row: lanes stride the row (coalesced) and a fixed xor-shuffle tree sums
@@ -336,10 +336,12 @@ NHQG_DEV double wave_sum(double v) {
 NHQG_DEV void block_matvec(int n_out, int n_in, const double* __restrict__ M, const double* x,
                            double* y) {
   // SBP_MV_ROWS rows per wavefront at a time, so that the row loads of a
-  // round are all in flight together (the matrices come from L2 / HBM)
+  // round are all in flight together (the matrices come from L2 / HBM);
+  // the grid's wavefronts share the rows
   constexpr int R = SBP_MV_ROWS;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int i0 = wave * R; i0 < n_out; i0 += nw * R) {
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const int gw = blockIdx.x * nw + (threadIdx.x >> 6), ngw = gridDim.x * nw;
+  for (int i0 = gw * R; i0 < n_out; i0 += ngw * R) {
     double t[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) t[r] = 0.0;
@@ -514,95 +516,146 @@ sbp_rhs_kernel(int n, const double* __restrict__ Bg, const double* __restrict__ 
   XY[idx] = r;
 }
 
-// Banded solve of every column of X|Y with the preamble's LU, one thread
-// per column (64 per block).  Inputs stream from global memory in chunks of
-// SBP_CH rows, the next chunk loaded before the current one runs through
-// the dependent recurrence; intermediate values live in LDS and the column
-// is written back once at the end, so no global store sits in front of a
-// load wait (stores and loads share the vmcnt counter).
+// Banded solve of every column of X|Y with the preamble's LU, one column
+// per lane (64 per block).  The recurrences are serial in the row, so the
+// block's four waves first stage its 64 right-hand-side columns and the LU
+// bands in LDS (wave w loads rows w, w + 4, ..., two batches of SBP_LB rows
+// in flight per lane), wave 0 alone runs the forward and backward
+// substitutions from LDS (the next chunk of SBP_CH rows read ahead of the
+// current chunk's chain), and the four waves write the columns back.  Same
+// operations in the same order as a plain row loop: the result does not
+// depend on the staging.
 #define SBP_CH 8
-__global__ void __launch_bounds__(64)
+#define SBP_LB 16
+#define SBP_CW 4   // waves per block
+__global__ void __launch_bounds__(SBP_CW * 64)
 sbp_cols_kernel(int n, const double* __restrict__ LUg, double* __restrict__ XY) {
   __shared__ double cs[SBP_NMAX * 64];
-  const int t = threadIdx.x;
+  __shared__ double ba[SBP_NMAX], bb[SBP_NMAX], be[SBP_NMAX], bf[SBP_NMAX], brd[SBP_NMAX];
+  const int t = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = blockIdx.x * 64 + t;
   const int ld = 2 * n;
   const bool on = col < ld;
   const double* rdp = LUg + 5 * n;
   const double* cp = XY + min(col, ld - 1);
-  const int nch = (n + SBP_CH - 1) / SBP_CH;
-  // forward: y_i = r_i - (b_i y_{i-1} + a_i y_{i-2})
-  double rv[SBP_CH], av[SBP_CH], bv[SBP_CH], rn[SBP_CH], an[SBP_CH], bn[SBP_CH];
-  auto fload = [&](int c) {
+  for (int i = threadIdx.x; i < n; i += SBP_CW * 64) {
+    ba[i] = LUg[i * 5 + 0];
+    bb[i] = LUg[i * 5 + 1];
+    be[i] = LUg[i * 5 + 3];
+    bf[i] = LUg[i * 5 + 4];
+    brd[i] = rdp[i];
+  }
+  {
+    // this wave's rows i = wv + SBP_CW * j; batch b + 1 in flight while
+    // batch b is stored
+    const int nr = (n - wv + SBP_CW - 1) / SBP_CW;
+    double v0[SBP_LB], v1[SBP_LB];
+    auto load = [&](double (&v)[SBP_LB], int j0) {
 #pragma unroll
-    for (int u = 0; u < SBP_CH; ++u) {
-      const int i = min(c * SBP_CH + u, n - 1);
-      rn[u] = cp[(size_t)i * ld];
-      an[u] = LUg[i * 5 + 0];
-      bn[u] = LUg[i * 5 + 1];
+      for (int u = 0; u < SBP_LB; ++u)
+        v[u] = cp[(size_t)(wv + SBP_CW * min(j0 + u, nr - 1)) * ld];
+    };
+    auto store = [&](const double (&v)[SBP_LB], int j0) {
+#pragma unroll
+      for (int u = 0; u < SBP_LB; ++u)
+        if (j0 + u < nr) cs[(wv + SBP_CW * (j0 + u)) * 64 + t] = v[u];
+    };
+    const int nb = (nr + SBP_LB - 1) / SBP_LB;
+    if (nb > 0) load(v0, 0);
+    for (int b = 0; b < nb; b += 2) {
+      if (b + 1 < nb) load(v1, (b + 1) * SBP_LB);
+      store(v0, b * SBP_LB);
+      if (b + 2 < nb) load(v0, (b + 2) * SBP_LB);
+      if (b + 1 < nb) store(v1, (b + 1) * SBP_LB);
     }
-  };
-  fload(0);
-  double y1 = 0.0, y2 = 0.0;
-  for (int c = 0; c < nch; ++c) {
+  }
+  __syncthreads();
+  if (wv == 0) {
+    const int nch = (n + SBP_CH - 1) / SBP_CH;
+    // forward: y_i = r_i - (b_i y_{i-1} + a_i y_{i-2})
+    double rv[SBP_CH], av[SBP_CH], bv[SBP_CH], rn[SBP_CH], an[SBP_CH], bn[SBP_CH];
+    auto fload = [&](int c) {
 #pragma unroll
-    for (int u = 0; u < SBP_CH; ++u) {
-      rv[u] = rn[u];
-      av[u] = an[u];
-      bv[u] = bn[u];
-    }
-    if (c + 1 < nch) fload(c + 1);
+      for (int u = 0; u < SBP_CH; ++u) {
+        const int i = min(c * SBP_CH + u, n - 1);
+        rn[u] = cs[i * 64 + t];
+        an[u] = ba[i];
+        bn[u] = bb[i];
+      }
+    };
+    fload(0);
+    double y1 = 0.0, y2 = 0.0;
+    for (int c = 0; c < nch; ++c) {
 #pragma unroll
-    for (int u = 0; u < SBP_CH; ++u) {
-      const int i = c * SBP_CH + u;
-      if (i < n) {
+      for (int u = 0; u < SBP_CH; ++u) {
+        rv[u] = rn[u];
+        av[u] = an[u];
+        bv[u] = bn[u];
+      }
+      if (c + 1 < nch) fload(c + 1);
+      auto fstep = [&](int u, int i) {
         const double v = rv[u] - (bv[u] * y1 + av[u] * y2);
         cs[i * 64 + t] = v;
         y2 = y1;
         y1 = v;
+      };
+      // full chunks without a per-row bound test
+      if ((c + 1) * SBP_CH <= n) {
+#pragma unroll
+        for (int u = 0; u < SBP_CH; ++u) fstep(u, c * SBP_CH + u);
+      } else {
+#pragma unroll
+        for (int u = 0; u < SBP_CH; ++u)
+          if (c * SBP_CH + u < n) fstep(u, c * SBP_CH + u);
       }
     }
-  }
-  // backward: x_i = (y_i - e_i x_{i+1} - f_i x_{i+2}) / d_i
-  double ev[SBP_CH], fv[SBP_CH], dv[SBP_CH], en[SBP_CH], fn[SBP_CH], dn[SBP_CH];
-  auto bload = [&](int c) {
+    // backward: x_i = (y_i - e_i x_{i+1} - f_i x_{i+2}) / d_i
+    double ev[SBP_CH], fv[SBP_CH], dv[SBP_CH], en[SBP_CH], fn[SBP_CH], dn[SBP_CH];
+    auto bload = [&](int c) {
 #pragma unroll
-    for (int u = 0; u < SBP_CH; ++u) {
-      const int i = max(n - 1 - (c * SBP_CH + u), 0);
-      rn[u] = cs[i * 64 + t];   // y from the forward pass (all rows written)
-      en[u] = LUg[i * 5 + 3];
-      fn[u] = LUg[i * 5 + 4];
-      dn[u] = rdp[i];
-    }
-  };
-  bload(0);
-  double x1 = 0.0, x2 = 0.0;
-  for (int c = 0; c < nch; ++c) {
+      for (int u = 0; u < SBP_CH; ++u) {
+        const int i = max(n - 1 - (c * SBP_CH + u), 0);
+        rn[u] = cs[i * 64 + t];   // y from the forward pass (all rows written)
+        en[u] = be[i];
+        fn[u] = bf[i];
+        dn[u] = brd[i];
+      }
+    };
+    bload(0);
+    double x1 = 0.0, x2 = 0.0;
+    for (int c = 0; c < nch; ++c) {
 #pragma unroll
-    for (int u = 0; u < SBP_CH; ++u) {
-      rv[u] = rn[u];
-      ev[u] = en[u];
-      fv[u] = fn[u];
-      dv[u] = dn[u];
-    }
-    if (c + 1 < nch) bload(c + 1);
-#pragma unroll
-    for (int u = 0; u < SBP_CH; ++u) {
-      const int i = n - 1 - (c * SBP_CH + u);
-      if (i >= 0) {
+      for (int u = 0; u < SBP_CH; ++u) {
+        rv[u] = rn[u];
+        ev[u] = en[u];
+        fv[u] = fn[u];
+        dv[u] = dn[u];
+      }
+      if (c + 1 < nch) bload(c + 1);
+      auto bstep = [&](int u, int i) {
         const double v = (rv[u] - ev[u] * x1 - fv[u] * x2) * dv[u];
         cs[i * 64 + t] = v;
         x2 = x1;
         x1 = v;
+      };
+      if ((c + 1) * SBP_CH <= n) {
+#pragma unroll
+        for (int u = 0; u < SBP_CH; ++u) bstep(u, n - 1 - (c * SBP_CH + u));
+      } else {
+#pragma unroll
+        for (int u = 0; u < SBP_CH; ++u)
+          if (n - 1 - (c * SBP_CH + u) >= 0) bstep(u, n - 1 - (c * SBP_CH + u));
       }
     }
   }
+  __syncthreads();
   if (on)
-    for (int i = 0; i < n; ++i) XY[(size_t)i * ld + col] = cs[i * 64 + t];
+    for (int i = wv; i < n; i += SBP_CW) XY[(size_t)i * ld + col] = cs[i * 64 + t];
 }
 
 // Mall (2n x 2n, row-major) = [M1 M2; G1 G2].  Rows [0, n): s2thb Z XY as a
-// 32 x 32-tiled product through LDS (4 outputs per thread), the Y half
+// 32 x 32-tiled product through LDS (4 outputs per thread, the next k tile
+// in flight in registers while the current one is used), the Y half
 // scaled by -mu dt_s; rows [n, 2n): the banded D1 rows, one element per
 // thread.
 #define SBP_TILE 32
@@ -618,16 +671,31 @@ sbp_compose_kernel(int n, const double* __restrict__ XY, const double* __restric
     const int r0 = (blockIdx.x / ntc) * SBP_TILE, c0 = (blockIdx.x % ntc) * SBP_TILE;
     const int tc = tid % SBP_TILE, tr = tid / SBP_TILE;   // 8 row groups of 4 rows
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < n; k0 += SBP_TILE) {
-      for (int e = tid; e < SBP_TILE * SBP_TILE; e += 256) {
+    // the next k tile is loaded into registers while the current one is used
+    constexpr int EPT = SBP_TILE * SBP_TILE / 256;
+    double pa[EPT], pb[EPT];
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int u = 0; u < EPT; ++u) {
+        const int e = tid + 256 * u;
         const int a = e / SBP_TILE, b = e % SBP_TILE;
         const int r = r0 + a, k = k0 + b;
-        As[a][b] = (r < n && k < n) ? s2thb[(size_t)r * n + k] : 0.0;
+        pa[u] = (r < n && k < n) ? s2thb[(size_t)r * n + k] : 0.0;
         const int kb = k0 + a, cb = c0 + b;
         // Z: rows 0 and n-1 of XY do not contribute
-        Bs[a][b] = (kb > 0 && kb < n - 1 && cb < ld) ? XY[(size_t)kb * ld + cb] : 0.0;
+        pb[u] = (kb > 0 && kb < n - 1 && cb < ld) ? XY[(size_t)kb * ld + cb] : 0.0;
+      }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < n; k0 += SBP_TILE) {
+#pragma unroll
+      for (int u = 0; u < EPT; ++u) {
+        const int e = tid + 256 * u;
+        As[e / SBP_TILE][e % SBP_TILE] = pa[u];
+        Bs[e / SBP_TILE][e % SBP_TILE] = pb[u];
       }
       __syncthreads();
+      if (k0 + SBP_TILE < n) fetch(k0 + SBP_TILE);
 #pragma unroll 8
       for (int kk = 0; kk < SBP_TILE; ++kk) {
         const double bv = Bs[kk][tc];
@@ -717,7 +785,8 @@ hipError_t launch_sbp_maps(int n, const double* sbp, const double* thb2s, const 
   const double* LUg = sbp + 5 * (size_t)n;
   hipLaunchKernelGGL(sbp_rhs_kernel, dim3((2 * n * n + 255) / 256), dim3(256), 0, s, n, Bg, thb2s,
                      d1b, XY);
-  hipLaunchKernelGGL(sbp_cols_kernel, dim3((2 * n + 63) / 64), dim3(64), 0, s, n, LUg, XY);
+  hipLaunchKernelGGL(sbp_cols_kernel, dim3((2 * n + 63) / 64), dim3(SBP_CW * 64), 0, s, n, LUg,
+                     XY);
   const int ntr = (n + SBP_TILE - 1) / SBP_TILE, ntc = (2 * n + SBP_TILE - 1) / SBP_TILE;
   const int nband = (2 * n * n + 255) / 256;
   hipLaunchKernelGGL(sbp_compose_kernel, dim3(ntr * ntc + nband), dim3(256), 0, s, n, XY, s2thb,
@@ -733,8 +802,9 @@ hipError_t launch_sbp_apply(int n, int P, const double* partial, const double* M
   return hipGetLastError();
 }
 
-// y = M x for small dense M (n_out x n_in); one block.
-__global__ void __launch_bounds__(1024)
+// y = M x for small dense M (n_out x n_in <= SBP_NMAX), SBP_MV_ROWS rows
+// per wavefront over many blocks.
+__global__ void __launch_bounds__(256)
 small_matvec_kernel(int n_out, int n_in, const double* __restrict__ M, const double* __restrict__ x,
                     double* __restrict__ y) {
   block_matvec(n_out, n_in, M, x, y);
@@ -748,7 +818,9 @@ small_axpy_kernel(int n, const double* x0, const double* x1, const double* x2, d
 
 hipError_t launch_small_matvec(int n_out, int n_in, const double* M, const double* x, double* y,
                                hipStream_t s) {
-  hipLaunchKernelGGL(small_matvec_kernel, dim3(1), dim3(1024), 0, s, n_out, n_in, M, x, y);
+  if (n_in > SBP_NMAX) return hipErrorInvalidValue;
+  const int nb = (n_out + 4 * SBP_MV_ROWS - 1) / (4 * SBP_MV_ROWS);
+  hipLaunchKernelGGL(small_matvec_kernel, dim3(nb), dim3(256), 0, s, n_out, n_in, M, x, y);
   return hipGetLastError();
 }
 
