@@ -354,7 +354,7 @@ int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
 // the return value flags the fields done that way (bit f).
 int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, double* Eq,
                  double* Ew, double* Eth, double* const* R = nullptr, double al = 0.0,
-                 int* fused = nullptr) {
+                 int* fused = nullptr, const Epi* th_tail = nullptr, double th_scale = 0.0) {
   const Ops& o = c->o;
   RUN(gemm_jobs(c, {Job{&o.V, q, c->qn, epi_make(1.0)}, Job{&o.VSw, w, c->wn, epi_make(1.0)},
                     Job{&o.VSd, th, c->thn, epi_make(1.0)}}));
@@ -406,6 +406,17 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
     epi_add(eq, -1.0, c->spq);
     epi_add(ew, -1.0, c->spw);
     epi_add(et, -1.0, c->spth);
+  }
+  if (th_tail) {
+    // Eth receives th_scale * E_th + (the tail's terms) instead of E_th
+    if (et.nX + th_tail->nX > NHQG_EPI_TERMS) {
+      c->err = "too many epilogue terms";
+      return NHQG_ERR_ARG;
+    }
+    et.a_acc *= th_scale;
+    for (int t = 0; t < et.nX; ++t) et.aX[t] *= th_scale;
+    for (int t = 0; t < th_tail->nX; ++t)
+      epi_add(et, th_tail->aX[t], th_tail->X[t], th_tail->sX[t]);
   }
   Job jq{&o.Vinv, c->qn, Eq, eq}, jw{&o.Pw, c->wn, Ew, ew}, jt{&o.Pd, c->thn, Eth, et};
   if (fused) *fused = 0;
@@ -594,8 +605,22 @@ int enqueue_step_body(nhqg_ctx* c) {
     th1 = c->c1th;
     thb1 = c->c1thb;
   }
-  // stage 2
-  RUN(explicit_rhs(c, c->s1q, c->s1w, th1, c->E2q, c->E2w, c->E2th));
+  // stage 2.  With a Dirichlet w (I_th = w1), R_th2 below comes out of the
+  // theta back transform: dt(1-delta) E_th2 plus the other terms
+  const bool rth_fused = !c->P.w_neumann;
+  Epi htail = epi_make(0.0);
+  if (rth_fused) {
+    epi_add(htail, 1.0, c->yth);
+    epi_add(htail, dt * delta, c->E1th);
+    epi_add(htail, om, c->s1w);
+    epi_add(htail, -om, th1, o.diss_th);
+    if (em) {
+      epi_add(htail, om / al, th1);
+      epi_add(htail, -om / al, c->s1th);
+    }
+  }
+  RUN(explicit_rhs(c, c->s1q, c->s1w, th1, c->E2q, c->E2w, rth_fused ? c->Rth : c->E2th, nullptr,
+                   0.0, nullptr, rth_fused ? &htail : nullptr, dt * (1.0 - delta)));
   {
     // R_q2 = y + dt(delta E1 + (1-delta) E2) + om*(G_Z S_w w1) - om*diss_q*q1
     Epi e = epi_make(om);
@@ -618,22 +643,24 @@ int enqueue_step_body(nhqg_ctx* c) {
     epi_add(f, -om, c->s1w, o.diss_w);
     RUN(gemm_jobs(c, {Job{&o.GSw, c->s1w, c->Rq, e}, Job{&o.Qw, c->s1q, c->Rw, f}}));
     // R_th2 = ... + om*I_th - om diss_th th1 + om*(th1 - th1p)/alpha
-    const double* ith = c->s1w;
-    if (c->P.w_neumann) {
-      RUN(gemm(c, o.Mwt, c->s1w, c->tmpg, epi_make(1.0)));
-      ith = c->tmpg;
+    if (!rth_fused) {
+      const double* ith = c->s1w;
+      if (c->P.w_neumann) {
+        RUN(gemm(c, o.Mwt, c->s1w, c->tmpg, epi_make(1.0)));
+        ith = c->tmpg;
+      }
+      Epi h = epi_make(0.0);
+      epi_add(h, 1.0, c->yth);
+      epi_add(h, dt * delta, c->E1th);
+      epi_add(h, dt * (1.0 - delta), c->E2th);
+      epi_add(h, om, ith);
+      epi_add(h, -om, th1, o.diss_th);
+      if (em) {
+        epi_add(h, om / al, th1);
+        epi_add(h, -om / al, c->s1th);
+      }
+      RUN(combine(c, Ng, c->Rth, h));
     }
-    Epi h = epi_make(0.0);
-    epi_add(h, 1.0, c->yth);
-    epi_add(h, dt * delta, c->E1th);
-    epi_add(h, dt * (1.0 - delta), c->E2th);
-    epi_add(h, om, ith);
-    epi_add(h, -om, th1, o.diss_th);
-    if (em) {
-      epi_add(h, om / al, th1);
-      epi_add(h, -om / al, c->s1th);
-    }
-    RUN(combine(c, Ng, c->Rth, h));
     if (em)
       PL(CL_SMALL, 0.0, launch_small_axpy(Nq, c->ythb, thb1, c->s1thb, om / al, c->Rthb, c->st));
     else

@@ -435,38 +435,44 @@ sbp_prepare_kernel(int n, int P, const double* __restrict__ partial, const doubl
         bk3[u] = LU[r3 * 5 + 1];
         dk3[u] = LU[r3 * 5 + 2];
       }
-#pragma unroll
-      for (int u = 0; u < C; ++u) {
-        const int k = k0 + u;
-        if (k < n - 1) {
-          const double r = 1.0 / d0;
-          rd[k] = r;
-          const double m1 = b1 * r;
-          LU[(k + 1) * 5 + 1] = m1;
-          const double nd1 = d1 - m1 * e0;
-          const double ne1 = e1 - m1 * f0;
-          double nb2 = 0.0, nd2 = 0.0;
-          if (k + 2 < n) {
-            const double m2 = a2 * r;
-            LU[(k + 2) * 5 + 0] = m2;
-            nb2 = b2 - m2 * e0;
-            nd2 = d2 - m2 * f0;
-          }
-          LU[(k + 1) * 5 + 2] = nd1;
-          LU[(k + 1) * 5 + 3] = ne1;
-          // shift the window: row k+1 becomes the pivot row
-          d0 = nd1;
-          e0 = ne1;
-          f0 = fk1[u];
-          b1 = nb2;
-          d1 = nd2;
-          e1 = (k + 2 < n) ? ek2[u] : 0.0;
-          if (k + 3 < n) {
-            a2 = ak3[u];
-            b2 = bk3[u];
-            d2 = dk3[u];
-          }
+      // one elimination step; chk = false where k + 3 < n is known
+      auto step = [&](int u, int k, bool chk) {
+        const double r = 1.0 / d0;
+        rd[k] = r;
+        const double m1 = b1 * r;
+        LU[(k + 1) * 5 + 1] = m1;
+        const double nd1 = d1 - m1 * e0;
+        const double ne1 = e1 - m1 * f0;
+        double nb2 = 0.0, nd2 = 0.0;
+        if (!chk || k + 2 < n) {
+          const double m2 = a2 * r;
+          LU[(k + 2) * 5 + 0] = m2;
+          nb2 = b2 - m2 * e0;
+          nd2 = d2 - m2 * f0;
         }
+        LU[(k + 1) * 5 + 2] = nd1;
+        LU[(k + 1) * 5 + 3] = ne1;
+        // shift the window: row k+1 becomes the pivot row
+        d0 = nd1;
+        e0 = ne1;
+        f0 = fk1[u];
+        b1 = nb2;
+        d1 = nd2;
+        e1 = (!chk || k + 2 < n) ? ek2[u] : 0.0;
+        if (!chk || k + 3 < n) {
+          a2 = ak3[u];
+          b2 = bk3[u];
+          d2 = dk3[u];
+        }
+      };
+      // chunks away from the last rows run without bound tests
+      if (k0 + C + 3 <= n) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) step(u, k0 + u, false);
+      } else {
+#pragma unroll
+        for (int u = 0; u < C; ++u)
+          if (k0 + u < n - 1) step(u, k0 + u, true);
       }
     }
     rd[n - 1] = 1.0 / d0;
