@@ -145,14 +145,18 @@ int upload_op(nhqg_ctx* c, GemmOp* op, const double* host, int M, int K, int mt)
 // Reflection-split halves of an uploaded operator (gemm_split.hip), unless
 // dense contractions are asked for or the operator lacks the structure;
 // prefer_cols picks the column butterfly for operators that have both (the
-// CGL transform), so that it shares a batch with its neighbours.
-int upload_op_split(nhqg_ctx* c, GemmOp* op, const double* host, bool prefer_cols) {
+// CGL transform), so that it shares a batch with its neighbours; with_update
+// marks the operators applied with the fused SBP substep update, whose
+// kernel runs shallower k-chunks.
+int upload_op_split(nhqg_ctx* c, GemmOp* op, const double* host, bool prefer_cols,
+                    bool with_update = false) {
   if (!host || c->P.dense_contractions) return NHQG_OK;
   // every operator of a context gets the tile count of the largest half
   // (its Nz/2 + 1 rows: the last one on the VALU when Nz/2 is a multiple of 16)
   const int min_mth = ((c->Nq - 1) / 2 + 15) / 16;
   std::vector<double> packed;
-  if (!gemm_split_plan(host, op->M, op->K, prefer_cols, 1e-11, min_mth, *op, packed))
+  if (!gemm_split_plan(host, op->M, op->K, prefer_cols, 1e-11, min_mth,
+                       gemm_split_kc(with_update), *op, packed))
     return NHQG_OK;
   op->sp_pref = prefer_cols;
   double* d = nullptr;
@@ -169,7 +173,8 @@ int upload_op_i8(nhqg_ctx* c, GemmOp* op, const double* host, int S) {
     GemmOp plan = *op;
     std::vector<double> packed;
     const int min_mth = op->sp_mth;
-    if (gemm_split_plan(host, op->M, op->K, op->sp_pref != 0, 1e-11, min_mth, plan, packed) &&
+    if (gemm_split_plan(host, op->M, op->K, op->sp_pref != 0, 1e-11, min_mth, op->sp_kc, plan,
+                        packed) &&
         plan.sp_mth == op->sp_mth) {
       std::vector<int8_t> a8;
       std::vector<double> sa, al;
@@ -896,7 +901,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(upload_op_split(c, &o.GSw, h->GSw, false));
   RUN(upload_op_split(c, &o.W2S, h->W2S, false));
   RUN(upload_op_split(c, &o.T2S, h->T2S, false));
-  RUN(upload_op_split(c, &o.S2T, h->S2T, true));
+  RUN(upload_op_split(c, &o.S2T, h->S2T, true, true));
   {
     // one SBP substep cycle: Psbp = T2S * S2T (Nq x Nq)
     std::vector<double> P((size_t)Nq * Nq, 0.0);
@@ -906,7 +911,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
         for (int j = 0; j < Nq; ++j) P[(size_t)i * Nq + j] += a * h->S2T[(size_t)k * Nq + j];
       }
     RUN(upload_op(c, &o.Psbp, P.data(), Nq, Nq, mt));
-    RUN(upload_op_split(c, &o.Psbp, P.data(), false));
+    RUN(upload_op_split(c, &o.Psbp, P.data(), false, true));
   }
   RUN(upload(c, &o.thb2s, h->thb2s, (size_t)Nq * Nq));
   RUN(upload(c, &o.s2thb, h->s2thb, (size_t)Nq * Nq));

@@ -218,6 +218,7 @@ struct GemmOp {
   int sp_nk[2] = {0, 0};
   int sp_nbi = 0, sp_nbo = 0;
   int sp_pref = 0;   // the prefer_cols the split was planned with
+  int sp_kc = 0;     // the k-chunk sp_kp is padded to (gemm_split_kc)
   // int8 slices of the two halves (gemm_i8.hip, split kernel): tile t, half
   // h, slice p, k-step kk at (((t*2 + h)*S + p)*snkh + kk)*64 + lane (v4i);
   // ssa / safirst at h*snti*16 + r; snti row tiles per half, snkh k-steps
@@ -257,9 +258,12 @@ hipError_t launch_gemm_batch(const GemmBatch& b, int nj, hipStream_t s);
 // fields of op (not sp_At) and the packed halves, or returns false when the
 // operator has none of the structures within tol * max|A|.  prefer_cols
 // tries the column butterfly before the row butterfly; the halves get at
-// least min_mth row tiles (operators that share a batch share one count).
+// least min_mth row tiles (operators that share a batch share one count)
+// and are padded to a multiple of kc rows of depth.
 bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol, int min_mth,
-                     GemmOp& op, std::vector<double>& packed);
+                     int kc, GemmOp& op, std::vector<double>& packed);
+// k-chunk of the split kernel with (upd) or without the fused substep update
+int gemm_split_kc(bool upd);
 // Every job of the batch split with the same sp_mth, sp_ib and sp_ob (and
 // all with or all without the substep update).
 bool gemm_split_batch_ok(const GemmBatch& b, int nj);

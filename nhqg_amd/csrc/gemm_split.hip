@@ -21,7 +21,9 @@
 // in acc1, MTH tiles of 16 each), A's halves staged through LDS in k-chunks
 // as in the dense kernel (gemm_f64.hip), B in a register ring of raw rows
 // that are combined (butterfly, substep update) when used, so the ring's
-// loads stay in flight.
+// loads stay in flight.  Chunks are 16 rows deep, 8 with the fused substep
+// update (its extra ring registers would not fit 2 waves per SIMD); each
+// operator's halves are padded to the chunk of the kernel it runs in.
 #include "common.h"
 
 #include <algorithm>
@@ -31,7 +33,10 @@
 #define SP_NW 4   // waves per workgroup (two workgroups per CU)
 #endif
 #ifndef SP_KC
-#define SP_KC 8   // k-chunk staged in LDS per iteration (double buffered)
+#define SP_KC 16  // k-chunk staged in LDS per iteration (double buffered)
+#endif
+#ifndef SP_KCU
+#define SP_KCU 8  // the same with the fused substep update
 #endif
 #ifndef SP_R
 #define SP_R 2    // B ring depth in chunks
@@ -84,10 +89,10 @@ NHQG_DEV void split_epilogue(const GemmJob& J, const v4d (&acc0)[MTH], const v4d
 }
 
 template <int MTH, bool IB, bool OB, bool UPD>
-__global__ void __launch_bounds__(SP_NW * 64)
+__global__ void __launch_bounds__(SP_NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 gemm_split_kernel(GemmBatch bt) {
   constexpr int NT = SP_NW * 64;
-  constexpr int KC = SP_KC;
+  constexpr int KC = UPD ? SP_KCU : SP_KC;
   constexpr int R = SP_R;
   constexpr int KS = KC / 4;
   constexpr int MP = MTH * 16;       // rows per half
@@ -259,8 +264,11 @@ int half_of(const Side& s, int i) {
 
 }  // namespace
 
+int gemm_split_kc(bool upd) { return upd ? SP_KCU : SP_KC; }
+
 bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol, int min_mth,
-                     GemmOp& op, std::vector<double>& packed) {
+                     int kc, GemmOp& op, std::vector<double>& packed) {
+  if (kc < 1) return false;
   if (M < 8 || K < 8 || M > 2 * 16 * SP_MAXH) return false;
   double mx = 0.0;
   for (size_t i = 0; i < (size_t)M * K; ++i) mx = std::max(mx, std::fabs(A[i]));
@@ -378,7 +386,7 @@ bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol
   }
   if (mth > SP_MAXH) return false;
   const int MP = 16 * mth, MW = 2 * MP + 16;
-  const int kp = (std::max(nk[0], nk[1]) + SP_KC - 1) / SP_KC * SP_KC;
+  const int kp = (std::max(nk[0], nk[1]) + kc - 1) / kc * kc;
 
   // E/O entries
   const int Nr = M - 1, Nc = K - 1;
@@ -409,6 +417,7 @@ bool gemm_split_plan(const double* A, int M, int K, bool prefer_cols, double tol
   op.sp_mth = mth;
   op.sp_xrow = xrow;
   op.sp_kp = kp;
+  op.sp_kc = kc;
   op.sp_ib = ib;
   op.sp_ob = ob;
   op.sp_istr = ib ? 0 : cs.str;
@@ -436,6 +445,7 @@ bool gemm_split_batch_ok(const GemmBatch& b, int nj) {
         (b.job[j].W != nullptr) != upd || !b.job[j].B || !b.job[j].C)
       return false;
     if (upd && (!p.sp_ib || !b.job[j].g)) return false;
+    if (p.sp_kp % gemm_split_kc(upd) != 0) return false;   // padded for the other kernel
   }
   return o.sp_mth >= 1 && o.sp_mth <= SP_MAXH;
 }
