@@ -44,6 +44,10 @@ struct nhqg_ctx {
   int Pred = 0;
   size_t bytes = 0;
   hipStream_t st = nullptr;
+  // side stream of the step (default priority, like st): the SBP corrector's
+  // preamble runs there, forked from st by ev_fork and joined by ev_join
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -457,9 +461,10 @@ double shell_bytes(const nhqg_ctx* c) {
   return 8.0 * c->P.n_shells * np * np + 4.0 * 16.0 * c->Ng * (double)c->nm;
 }
 
-// imex_implicit_solve (nhqg/solver.py:691-738), q_boundary='none'.
-int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double* Rth,
-                   double* oq, double* ow, double* oth) {
+// imex_implicit_solve (nhqg/solver.py:691-738), q_boundary='none', in two
+// parts: w and theta (implicit_solve_wth), then q from w (implicit_solve_q).
+int implicit_solve_wth(nhqg_ctx* c, const double* Rq, const double* Rw, const double* Rth,
+                       double* ow, double* oth) {
   const Ops& o = c->o;
   const double al = c->P.gamma_imex * c->P.dt;
   const double ars = al * c->P.ra_sigma;
@@ -502,12 +507,32 @@ int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double
     et.s_out = o.iath;
     RUN(gemm(c, o.Mwt, ow, oth, et));
   }
+  return NHQG_OK;
+}
+
+int implicit_solve_q(nhqg_ctx* c, const double* Rq, const double* ow, double* oq) {
+  const Ops& o = c->o;
+  const double al = c->P.gamma_imex * c->P.dt;
   Epi eq = epi_make(al);
   epi_add(eq, 1.0, Rq);
   eq.s_out = o.iaq;
   RUN(gemm(c, o.GSw, ow, oq, eq));
   return NHQG_OK;
 }
+
+int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double* Rth,
+                   double* oq, double* ow, double* oth) {
+  RUN(implicit_solve_wth(c, Rq, Rw, Rth, ow, oth));
+  return implicit_solve_q(c, Rq, ow, oq);
+}
+
+// Launches in the scope of a SideStream go to the side stream st2 (the
+// helpers all launch on c->st); the destructor restores st on every path.
+struct SideStream {
+  nhqg_ctx* c;
+  explicit SideStream(nhqg_ctx* x) : c(x) { std::swap(c->st, c->st2); }
+  ~SideStream() { std::swap(c->st, c->st2); }
+};
 
 // SBP2 thermal corrector with hoisted invariants (nhqg/solver.py:909-950).
 // th_out may equal th_in; thb_out is written (thb_in copied first).
@@ -516,17 +541,24 @@ int implicit_solve(nhqg_ctx* c, const double* Rq, const double* Rw, const double
 // S2T then T2S), and the update th - sdt * w_sbp * g with zeroed boundary
 // rows is applied as the contraction reads its input; only the last substep
 // returns to Galerkin coefficients (S2T).
-int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double* thb_in,
-              double* th_out, double* thb_out, double total_dt) {
+//
+// The preamble (w_sbp = W2S w, <w_sbp^2>, the factorisation and the substep
+// maps) depends on w alone.  corrector_preamble_fork enqueues it on the side
+// stream as soon as w is final, so that it runs beside the launches that
+// precede the corrector; corrector(..., forked = true) then joins it before
+// the first substep.
+int corrector_preamble(nhqg_ctx* c, const double* w_in, double total_dt, bool with_t2s,
+                       const double* th_in) {
   const Ops& o = c->o;
   const int n = c->Nq;
   const int ns = c->P.substeps > 0 ? c->P.substeps : 1;
   const double sdt = total_dt / ns;
   const double mu = c->P.eps_sq, kappa = 1.0 / c->P.sigma;
-  double* cur = c->thsbp;
-  double* nxt = c->thsbp2;
-  RUN(gemm_jobs(c, {Job{&o.W2S, w_in, c->wsbp, epi_make(1.0)},
-                    Job{&o.T2S, th_in, cur, epi_make(1.0)}}));
+  if (with_t2s)
+    RUN(gemm_jobs(c, {Job{&o.W2S, w_in, c->wsbp, epi_make(1.0)},
+                      Job{&o.T2S, th_in, c->thsbp, epi_make(1.0)}}));
+  else
+    RUN(gemm_jobs(c, {Job{&o.W2S, w_in, c->wsbp, epi_make(1.0)}}));
   PL(CL_PARSEVAL, 16.0 * n * (double)c->nm,
      launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->wsbp, o.pw, o.partner, c->partial,
                      c->st));
@@ -534,6 +566,39 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
                         0.25 * mu * sdt * sdt, c->sbp, c->st));
   PL(CL_SBP, 0.0, launch_sbp_maps(n, c->sbp, o.thb2s, o.s2thb, o.d1b, mu * sdt, c->sbp_xy,
                                   c->sbp_map, c->st));
+  return NHQG_OK;
+}
+
+// The side-stream preamble (not in a profiled step, whose classes are timed
+// on one stream).  Returns whether it forked.
+int corrector_preamble_fork(nhqg_ctx* c, const double* w_in, double total_dt, bool* forked) {
+  *forked = false;
+  if (c->prof || !c->st2) return NHQG_OK;
+  LK(hipEventRecord(c->ev_fork, c->st));
+  LK(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+  {
+    SideStream side(c);
+    RUN(corrector_preamble(c, w_in, total_dt, false, nullptr));
+    LK(hipEventRecord(c->ev_join, c->st));
+  }
+  *forked = true;
+  return NHQG_OK;
+}
+
+int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double* thb_in,
+              double* th_out, double* thb_out, double total_dt, bool forked = false) {
+  const Ops& o = c->o;
+  const int n = c->Nq;
+  const int ns = c->P.substeps > 0 ? c->P.substeps : 1;
+  const double sdt = total_dt / ns;
+  double* cur = c->thsbp;
+  double* nxt = c->thsbp2;
+  if (forked) {
+    RUN(gemm(c, o.T2S, th_in, cur, epi_make(1.0)));
+    LK(hipStreamWaitEvent(c->st, c->ev_join, 0));
+  } else {
+    RUN(corrector_preamble(c, w_in, total_dt, true, th_in));
+  }
   // th_bar ping-pongs between thb_out and a scratch level vector; an even
   // number of substeps ends in thb_out (an odd one gets a final copy)
   double* tb_a = c->thbt;
@@ -600,13 +665,17 @@ int enqueue_step_body(nhqg_ctx* c) {
       RUN(combine(c, nl[f], R1[f], e));
     }
   }
-  RUN(implicit_solve(c, c->Rq, c->Rw, c->Rth, c->s1q, c->s1w, c->s1th));
+  // the corrector's preamble forks as soon as w1 is final
+  bool fk = false;
+  RUN(implicit_solve_wth(c, c->Rq, c->Rw, c->Rth, c->s1w, c->s1th));
+  if (em) RUN(corrector_preamble_fork(c, c->s1w, al, &fk));
+  RUN(implicit_solve_q(c, c->Rq, c->s1w, c->s1q));
   PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->ythb, c->s1thb, c->st));
   // stage 1 corrector
   const double* th1 = c->s1th;
   const double* thb1 = c->s1thb;
   if (em) {
-    RUN(corrector(c, c->s1w, c->s1th, c->s1thb, c->c1th, c->c1thb, al));
+    RUN(corrector(c, c->s1w, c->s1th, c->s1thb, c->c1th, c->c1thb, al, fk));
     th1 = c->c1th;
     thb1 = c->c1thb;
   }
@@ -671,9 +740,11 @@ int enqueue_step_body(nhqg_ctx* c) {
     else
       PL(CL_SMALL, 0.0, launch_small_axpy(Nq, c->ythb, c->ythb, c->ythb, 0.0, c->Rthb, c->st));
   }
-  RUN(implicit_solve(c, c->Rq, c->Rw, c->Rth, c->yq, c->yw, c->yth));
+  RUN(implicit_solve_wth(c, c->Rq, c->Rw, c->Rth, c->yw, c->yth));
+  if (em) RUN(corrector_preamble_fork(c, c->yw, al, &fk));
+  RUN(implicit_solve_q(c, c->Rq, c->yw, c->yq));
   PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->ythb, c->st));
-  if (em) RUN(corrector(c, c->yw, c->yth, c->ythb, c->yth, c->ythb, al));
+  if (em) RUN(corrector(c, c->yw, c->yth, c->ythb, c->yth, c->ythb, al, fk));
   RUN(sanitize(c));
   return NHQG_OK;
 }
@@ -872,6 +943,9 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   c->ldc = c->nm_pad;
   c->Pred = 16;
   CK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  CK(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+  CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   CK(hipEventCreate(&c->ev0));
   CK(hipEventCreate(&c->ev1));
   const int Nq = c->Nq, Ng = c->Ng, nm = c->nm;
@@ -1305,12 +1379,16 @@ int nhqg_create(const nhqg_params* p, const nhqg_operators* ops, nhqg_ctx** out)
 int nhqg_destroy(nhqg_ctx* c) {
   if (!c) return NHQG_OK;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->st2) (void)hipStreamSynchronize(c->st2);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->cl_w) (void)hipFree(c->cl_w);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return NHQG_OK;
