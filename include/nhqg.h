@@ -15,7 +15,9 @@
  * Conventions: plain pointers and sizes; all arrays row-major; complex
  * arrays are interleaved float64 pairs (numpy complex128).  Every call
  * returns an NHQG_* status; nothing aborts.  A context owns one HIP stream
- * on the current device and is not thread safe.
+ * (the context stream) on the current device, plus a side stream that the
+ * step forks from it and joins back within each call, and is not thread
+ * safe.
  */
 #ifndef NHQG_H
 #define NHQG_H
