@@ -50,41 +50,48 @@ constexpr int SP_MAXH = 9;   // row tiles per half: M <= 2 * 144
 
 // Epilogue of tile T and up.  v_mfma_f64_16x16x4_f64 output: D[i][j] of a
 // tile sits in lane j + 16*(i%4), register i/4, so a half's row is
-// rr = T*16 + r*4 + lk.  Tiles go by compile-time recursion: an epilogue
-// loop too large to unroll would index the accumulators at run time and
-// move them to scratch memory.
+// rr = T*16 + r*4 + lk.  Tiles go two at a time (16 values per epi_rows, so
+// each fused term costs one memory round trip per tile pair) by
+// compile-time recursion: an epilogue loop too large to unroll would index
+// the accumulators at run time and move them to scratch memory.
 template <int T, int MTH, bool OB>
 NHQG_DEV void split_epilogue(const GemmJob& J, const v4d (&acc0)[MTH], const v4d (&acc1)[MTH],
                              int ld, int col, int lk) {
   if constexpr (T < MTH) {
+    constexpr int NT = (T + 1 < MTH) ? 2 : 1;   // tiles in this call
+    constexpr int NV = 8 * NT;
     const int mode = col >> 1;
     const int nbo = J.op.sp_nbo, hbo = nbo >> 1;
-    double val[8];
-    int row[8];
-    bool ok[8];
+    double val[NV];
+    int row[NV];
+    bool ok[NV];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int rr = T * 16 + r * 4 + lk;
-      const double E = acc0[T][r], O = acc1[T][r];
-      if constexpr (OB) {
-        val[r] = E + O;
-        row[r] = rr;
-        ok[r] = rr <= hbo;
-        val[4 + r] = E - O;
-        row[4 + r] = nbo - rr;
-        ok[4 + r] = rr < hbo;
-      } else {
-        val[r] = E;
-        row[r] = J.op.sp_ostr * rr + J.op.sp_ooff[0];
-        ok[r] = rr < J.op.sp_nr[0];
-        val[4 + r] = O;
-        row[4 + r] = J.op.sp_ostr * rr + J.op.sp_ooff[1];
-        ok[4 + r] = rr < J.op.sp_nr[1];
+    for (int u = 0; u < NT; ++u) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = (T + u) * 16 + r * 4 + lk;
+        const double E = acc0[T + u][r], O = acc1[T + u][r];
+        const int a = 8 * u + r, b = 8 * u + 4 + r;
+        if constexpr (OB) {
+          val[a] = E + O;
+          row[a] = rr;
+          ok[a] = rr <= hbo;
+          val[b] = E - O;
+          row[b] = nbo - rr;
+          ok[b] = rr < hbo;
+        } else {
+          val[a] = E;
+          row[a] = J.op.sp_ostr * rr + J.op.sp_ooff[0];
+          ok[a] = rr < J.op.sp_nr[0];
+          val[b] = O;
+          row[b] = J.op.sp_ostr * rr + J.op.sp_ooff[1];
+          ok[b] = rr < J.op.sp_nr[1];
+        }
       }
     }
-    if (J.C2) epi_rows<8>(J.e2, J.C2, val, row, ok, ld, col, mode);
-    epi_rows<8>(J.e, J.C, val, row, ok, ld, col, mode);
-    split_epilogue<T + 1, MTH, OB>(J, acc0, acc1, ld, col, lk);
+    if (J.C2) epi_rows<NV>(J.e2, J.C2, val, row, ok, ld, col, mode);
+    epi_rows<NV>(J.e, J.C, val, row, ok, ld, col, mode);
+    split_epilogue<T + NT, MTH, OB>(J, acc0, acc1, ld, col, lk);
   }
 }
 
