@@ -71,7 +71,9 @@ template <int NV>
 NHQG_DEV void epi_rows(const Epi& e, double* C, const double (&val)[NV], const int (&row)[NV],
                        const bool (&ok)[NV], int ld, int col, int mode) {
   double o[NV];
+  // the per-mode scales load with the first term, not after the last
   const double sa = e.s_acc ? e.s_acc[mode] : 1.0;
+  const double so = e.s_out ? e.s_out[mode] : 1.0;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     o[i] = e.a_acc * val[i];
@@ -92,7 +94,6 @@ NHQG_DEV void epi_rows(const Epi& e, double* C, const double (&val)[NV], const i
       o[i] += w;
     }
   }
-  const double so = e.s_out ? e.s_out[mode] : 1.0;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     if (ok[i]) {
