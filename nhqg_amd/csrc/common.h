@@ -79,14 +79,19 @@ NHQG_DEV void epi_rows(const Epi& e, double* C, const double (&val)[NV], const i
     o[i] = e.a_acc * val[i];
     if (e.s_acc) o[i] *= sa;
   }
+  // byte offsets of the NV elements, formed once for every term and the
+  // stores (fields stay below 4 GiB: Nz + 1 <= 257 rows of ld doubles)
+  unsigned bo[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) bo[i] = ((unsigned)(ok[i] ? row[i] : 0) * (unsigned)ld + col) * 8u;
   for (int x = 0; x < e.nX; ++x) {
-    const double* X = e.X[x];
+    const char* X = reinterpret_cast<const char*>(e.X[x]);
     const double ax = e.aX[x];
     const bool hs = e.sX[x] != nullptr;
     const double sx = hs ? e.sX[x][mode] : 1.0;
     double v[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = X[(size_t)(ok[i] ? row[i] : 0) * ld + col];
+    for (int i = 0; i < NV; ++i) v[i] = *reinterpret_cast<const double*>(X + bo[i]);
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       double w = ax * v[i];
@@ -100,7 +105,7 @@ NHQG_DEV void epi_rows(const Epi& e, double* C, const double (&val)[NV], const i
       double w = o[i];
       if (e.s_out) w *= so;
       if (e.d_tab) w *= e.d_tab[(size_t)row[i] * e.d_ld + mode];
-      C[(size_t)row[i] * ld + col] = w;
+      *reinterpret_cast<double*>(reinterpret_cast<char*>(C) + bo[i]) = w;
     }
   }
 }
