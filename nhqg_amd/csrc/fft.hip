@@ -519,14 +519,22 @@ row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, doubl
   double2* bb = buf[1][team];
   const size_t r0 = ((size_t)l * N + x0) * nky, r1 = r0 + nky;
   const double sc = 1.0 / ((double)N * (double)N);
+  // physical-space trap and sponge rows: loaded up front for N <= 256; at
+  // N >= 512 just before use, which keeps them out of the register peak
+  // (at N = 1024 holding them spilled 52 B per lane)
+  constexpr bool LATE = N >= 512;
   double e0[4], e1[4], s0[4], s1[4];
+  auto phys_rows = [&](const double* src, double (&p0)[4], double (&p1)[4]) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const size_t y = t + j * T;
-    e0[j] = TRAP ? eta_phys[(size_t)x0 * N + y] : 0.0;
-    e1[j] = TRAP ? eta_phys[(size_t)(x0 + 1) * N + y] : 0.0;
-    s0[j] = SPONGE ? sponge_phys[(size_t)x0 * N + y] : 0.0;
-    s1[j] = SPONGE ? sponge_phys[(size_t)(x0 + 1) * N + y] : 0.0;
+    for (int j = 0; j < 4; ++j) {
+      const size_t y = t + j * T;
+      p0[j] = src[(size_t)x0 * N + y];
+      p1[j] = src[(size_t)(x0 + 1) * N + y];
+    }
+  };
+  if (!LATE) {
+    if (TRAP) phys_rows(eta_phys, e0, e1);
+    if (SPONGE) phys_rows(sponge_phys, s0, s1);
   }
   auto store_pair = [&](int slot, const double2 (&A)[3], const double2 (&B)[3], bool iky) {
     if (!active) return;
@@ -553,6 +561,7 @@ row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, doubl
     double f0[4], f1[4];
     inv_pair_regs<N>(ba, bb, t, tw, hc, f0, f1, sc);
     if (TRAP && f == 0) {
+      if (LATE) phys_rows(eta_phys, e0, e1);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         f0[j] += e0[j];
@@ -578,6 +587,7 @@ row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, doubl
   }
   if (SPONGE) {
     // sigma times each coefficient level of q', w, theta (spectral.py:53-60)
+    if (LATE) phys_rows(sponge_phys, s0, s1);
 #pragma unroll 1
     for (int f = 0; f < 3; ++f) {
       if (f > 0 && l >= ng) break;  // block-uniform
