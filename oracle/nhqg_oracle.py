@@ -7,23 +7,31 @@ the HIP path; the product package (nhqg_amd) never imports it.
 It restates, in numpy/scipy float64, the reference algorithm read as text:
 
   operators       nhqg/grid.py:148-342, 349-432, 439-698
-  horizontal ops  nhqg/spectral.py:53-60, 260-358
-  step            nhqg/solver.py:45-155, 231-245, 346-358, 389-424,
-                  510-762, 873-1053, 1167-1242
-  diagnostics     nhqg/diagnostics.py:39-54, 206-226, 583-675
+  horizontal ops  nhqg/spectral.py:18-74, 187-358
+  step            nhqg/solver.py:45-155, 204-245, 346-374, 389-424,
+                  510-762, 873-1053, 1167-1242, 1249-1336
+  diagnostics     nhqg/diagnostics.py:39-110, 190-212, 334-369, 583-710
 
-Scope: ``horizontal_dealiasing='23_rule'``, ``vertical_dealiasing='none'``,
-``q_boundary='none'``, both ``w_bc_top`` values, both advection forms, trap
-and sponge, ``imex_scheme='ars222'`` with ``fixed_conduction`` or with
-``evolve_mean`` + ``balanced_sbp2_pc`` (the production branch), and
-``sbp_transfer_mode='interp'``.
+Scope: ``vertical_dealiasing='none'``, ``q_boundary='none'``, both
+``horizontal_dealiasing`` rules, both ``w_bc_top`` values, both advection
+forms, trap and sponge, ``imex_scheme='ars222'`` with ``fixed_conduction``,
+``evolve_mean`` + ``legacy`` exchange, or ``evolve_mean`` +
+``balanced_sbp2_pc`` (the production branch), ``sbp_transfer_mode=
+'interp'``; the RK4 validator for all three.
 
-Pinning (see tests/test_oracle_*.py): the reference's golden 20-step state
-(tests/golden/m2b_dirichlet_evolvemean_ref.npz) pins the whole step in the
-near-linear regime; analytic Jacobians, flux==Jacobian, numpy sponge product,
-analytic trap eta, closed-form operator identities, the eigenmode mean-T
-solve, dense per-mode block solves and Parseval identities pin the parts the
-fixture barely touches.  Functions with no such check: none on the step path.
+Pinning (tests/test_oracle_*.py): the reference's golden 20-step state
+pins the whole step in the near-linear regime; the reference's own tests
+restated here (mean-temperature eigenmode solve, analytic mean-flux RHS,
+Nyquist dealiasing of the mean flux, the heat flux diagnostic vs the solver
+flux path, the SBP exchange monitor, conservation of the nonlinear
+transfers, the heat-flux response of the mean) pin the legacy and
+diagnostic branches; analytic Jacobians, flux==Jacobian, numpy sponge
+product, analytic trap eta, closed-form operator identities, closed-form
+KE/max_speed of single Fourier modes, the SBP substep energy identity, the
+agreement of the balanced_sbp2_pc exchange with the legacy Chebyshev
+exchange as dt -> 0, Richardson ratios on q, theta and th_bar, dense
+per-mode block solves and Parseval identities pin the rest.  Functions with
+no such check: none on the step path or in the diagnostics.
 """
 
 from __future__ import annotations
@@ -209,8 +217,8 @@ def shell_inverses(ops, cfg):
 
 def build_operators(cfg, with_shells: bool = True):
     """Restatement of make_grid (grid.py:439-698) for the supported branch."""
-    if cfg.horizontal_dealiasing != "23_rule":
-        raise NotImplementedError("oracle covers horizontal_dealiasing='23_rule'")
+    if cfg.horizontal_dealiasing not in ("23_rule", "32_rule"):
+        raise ValueError(f"Unsupported horizontal_dealiasing={cfg.horizontal_dealiasing!r}")
     if cfg.vertical_dealiasing != "none":
         raise NotImplementedError("oracle covers vertical_dealiasing='none'")
     if cfg.q_boundary != "none":
@@ -219,7 +227,10 @@ def build_operators(cfg, with_shells: bool = True):
         raise NotImplementedError("oracle covers sbp_transfer_mode='interp'")
     Nx, N, L = cfg.Nx, cfg.Nz, cfg.L
     Nk = Nx // 2 + 1
-    o = SimpleNamespace(Nx=Nx, Nk=Nk, Nz=N, L=L)
+    # grid.Npad is 3Nx/2 under both rules (grid.py:444); the 2/3 rule uses it
+    # only for the diagnostics' and the legacy exchange's horizontal means
+    o = SimpleNamespace(Nx=Nx, Nk=Nk, Nz=N, L=L, Npad=3 * Nx // 2,
+                        rule=cfg.horizontal_dealiasing)
     xi = np.cos(np.pi * np.arange(N + 1) / N)
     o.xi, o.Z = xi, 0.5 * (1.0 + xi)
     o.cc = clenshaw_curtis(N)
@@ -275,8 +286,8 @@ def build_operators(cfg, with_shells: bool = True):
         eh = np.fft.rfft2(-0.5 * cfg.gamma * r ** 2 * trap)
         if np.sum(np.abs(eh * (1 - o.mask)) ** 2) > 1e-10 * np.sum(np.abs(eh) ** 2):
             raise ValueError("Trap background PV is under-resolved")
-        o.eta_hat = eh * o.mask
-    o.sponge_phys = None
+        o.eta_hat = eh * o.mask if o.rule == "23_rule" else eh
+    o.sponge_phys = o.sponge_phys_pad = None
     if cfg.sponge_rate < 0.0:
         raise ValueError("sponge_rate must be >= 0")
     if cfg.sponge_rate > 0.0:
@@ -290,7 +301,11 @@ def build_operators(cfg, with_shells: bool = True):
         sh = np.fft.rfft2(sig)
         if np.sum(np.abs(sh * (1 - o.mask)) ** 2) > 1e-10 * np.sum(np.abs(sh) ** 2):
             raise ValueError("Sponge profile is under-resolved")
-        o.sponge_phys = np.fft.irfft2(sh * o.mask, s=(Nx, Nx))
+        if o.rule == "23_rule":
+            sh = sh * o.mask
+        o.sponge_phys = np.fft.irfft2(sh, s=(Nx, Nx))
+        # sigma on the padded grid, zero-padded spectrum, no rescale (grid.py:663-667)
+        o.sponge_phys_pad = np.fft.irfft2(zero_pad(sh[None], Nx, o.Npad)[0], s=(o.Npad, o.Npad))
 
     if cfg.imex_scheme != "ars222":
         raise NotImplementedError("oracle covers imex_scheme='ars222'")
@@ -308,6 +323,18 @@ def build_operators(cfg, with_shells: bool = True):
     o.mu = cfg.mean_temp_eps_sq
     o.thermal_closure = cfg.thermal_closure
     o.exchange = cfg.mean_exchange_discretization
+    if o.exchange not in ("legacy", "balanced_sbp2_pc"):
+        raise NotImplementedError("oracle covers the legacy and balanced_sbp2_pc exchanges")
+    if o.thermal_closure not in ("fixed_conduction", "evolve_mean"):
+        raise ValueError(f"Unsupported thermal_closure={o.thermal_closure!r}")
+    o.sigma_mean = cfg.sigma
+    # evolve_mean tau solve of (I - alpha G_Z2) with the two Dirichlet rows
+    # (solver.py:746-758); kept as the matrix, applied with a dense solve
+    a_m = o.gamma_imex * cfg.dt * cfg.mean_temp_eps_sq / cfg.sigma
+    Am = np.eye(N + 1) - a_m * o.G_Z2
+    Am[N - 1, :] = ep
+    Am[N, :] = em
+    o.mean_solve_A = Am
     o.substeps = int(cfg.sbp_corrector_substeps)
     o.advection = cfg.nonlinear_advection
     o.cutoff = cfg.vertical_cutoff_n
@@ -341,6 +368,32 @@ def vmat(M: np.ndarray, f: np.ndarray) -> np.ndarray:
     return (M @ f.reshape(f.shape[0], -1)).reshape((M.shape[0],) + f.shape[1:])
 
 
+def zero_pad(f, Nx, Npad):
+    """Zero-pad (..., Nx, Nk) rfft2 spectra to (..., Npad, Npad//2+1): rows
+    0..Nx/2-1 keep their place, rows Nx/2..Nx-1 move to the top of the padded
+    kx range (spectral.py:18-34)."""
+    Nk = Nx // 2 + 1
+    out = np.zeros(f.shape[:-2] + (Npad, Npad // 2 + 1), dtype=np.complex128)
+    out[..., :Nx // 2, :Nk] = f[..., :Nx // 2, :]
+    out[..., Npad - Nx // 2:, :Nk] = f[..., Nx // 2:, :]
+    return out
+
+
+def truncate(fp, Nx, Npad):
+    """Inverse of zero_pad with the (Npad/Nx)^2 factor (spectral.py:37-50)."""
+    Nk = Nx // 2 + 1
+    keep = np.concatenate([fp[..., :Nx // 2, :Nk], fp[..., Npad - Nx // 2:, :Nk]], axis=-2)
+    return keep * (Npad / Nx) ** 2
+
+
+def padded_wavenumbers(o):
+    """Padded-grid wavenumbers from the base fundamental (spectral.py:80-88)."""
+    k0 = o.ky[0, 1] if o.Nk >= 2 else 0.0
+    kxp = k0 * np.fft.fftfreq(o.Npad, d=1.0 / o.Npad)
+    kyp = k0 * np.arange(o.Npad // 2 + 1)
+    return kxp[:, None], kyp[None, :]
+
+
 def flux23(psi, fields, o):
     """2/3-rule div(u f, v f) with u=-psi_y, v=psi_x on the Nx grid, output
     masked (spectral.py:307-328)."""
@@ -368,48 +421,120 @@ def jacobian23(psi, fields, o):
     return out
 
 
+def flux32(psi, fields, o):
+    """3/2-rule div(u f, v f): padded products, derivatives with the padded
+    wavenumbers, truncation (spectral.py:187-212)."""
+    Nx, Np = o.Nx, o.Npad
+    u = _irfft2(zero_pad(-1j * o.ky * psi, Nx, Np), Np)
+    v = _irfft2(zero_pad(1j * o.kx * psi, Nx, Np), Np)
+    kxp, kyp = padded_wavenumbers(o)
+    out = []
+    for f in fields:
+        fp = _irfft2(zero_pad(f, Nx, Np), Np)
+        d = 1j * kxp * _rfft2(u * fp) + 1j * kyp * _rfft2(v * fp)
+        out.append(truncate(d, Nx, Np))
+    return out
+
+
+def jacobian32(psi, fields, o):
+    """3/2-rule J[psi, f] with shared psi_x, psi_y (spectral.py:155-184)."""
+    Nx, Np = o.Nx, o.Npad
+    px = _irfft2(zero_pad(1j * o.kx * psi, Nx, Np), Np)
+    py = _irfft2(zero_pad(1j * o.ky * psi, Nx, Np), Np)
+    out = []
+    for f in fields:
+        fx = _irfft2(zero_pad(1j * o.kx * f, Nx, Np), Np)
+        fy = _irfft2(zero_pad(1j * o.ky * f, Nx, Np), Np)
+        out.append(truncate(_rfft2(px * fy - py * fx), Nx, Np))
+    return out
+
+
+def advection(psi, fields, o):
+    """_triple_horizontal_advection (solver.py:389-418)."""
+    if o.advection not in ("flux", "jacobian"):
+        raise ValueError(f"Unsupported nonlinear_advection={o.advection!r}")
+    if o.rule == "23_rule":
+        fn = flux23 if o.advection == "flux" else jacobian23
+    else:
+        fn = flux32 if o.advection == "flux" else jacobian32
+    return fn(psi, fields, o)
+
+
 def sponge23(f, o):
     """rfft2(sigma * irfft2(f)) * mask per level (spectral.py:53-60)."""
     return _rfft2(o.sponge_phys[None] * _irfft2(f, o.Nx)) * o.mask
 
 
-def hmean(a, b, o):
-    """<a b>_xy per level via Nx-grid irfft2 (solver.py:231-245)."""
-    return np.mean(_irfft2(a, o.Nx) * _irfft2(b, o.Nx), axis=(1, 2))
+def sponge32(f, o):
+    """Padded sigma*f, truncated (spectral.py:62-74)."""
+    fp = _irfft2(zero_pad(f, o.Nx, o.Npad), o.Npad)
+    return truncate(_rfft2(o.sponge_phys_pad[None] * fp), o.Nx, o.Npad)
 
 
-def explicit_rhs(s: OState, o) -> OState:
-    """explicit_rhs on the base grid used by the split/PC exchange and by
-    fixed_conduction runs (solver.py:510-599)."""
-    if o.thermal_closure == "evolve_mean" and o.exchange not in ("balanced_sbp2_pc",):
-        raise NotImplementedError("oracle covers evolve_mean only via balanced_sbp2_pc")
+def sponge_tendency(f, o):
+    """_sponge_tendency (solver.py:421-424)."""
+    return sponge23(f, o) if o.rule == "23_rule" else sponge32(f, o)
+
+
+def hmean(a, b, o, npad=None):
+    """<a b>_xy per level (horizontal_mean_from_nodal_spectral,
+    solver.py:231-245): irfft2 on the Nx grid when npad is None or Nx, else
+    on the zero-padded grid with both factors scaled by (Npad/Nx)^2."""
+    if npad is None or npad == o.Nx:
+        return np.mean(_irfft2(a, o.Nx) * _irfft2(b, o.Nx), axis=(1, 2))
+    sc = (npad / o.Nx) ** 2
+    ap = sc * _irfft2(zero_pad(a, o.Nx, npad), npad)
+    bp = sc * _irfft2(zero_pad(b, o.Nx, npad), npad)
+    return np.mean(ap * bp, axis=(1, 2))
+
+
+def _closure(o, base):
+    # base=True is the reference's base_grid inside balanced_sbp2_pc:
+    # fixed_conduction with the legacy exchange (solver.py:1173-1176)
+    return "fixed_conduction" if base else o.thermal_closure
+
+
+def explicit_rhs(s: OState, o, base: bool = False) -> OState:
+    """explicit_rhs (solver.py:510-599).  Under evolve_mean the legacy
+    exchange adds -dTheta_bar/dZ * w to E_theta and -eps^2 d<w theta>/dZ as
+    E_th_bar (solver.py:575-590); split exchanges (balanced_sbp2_pc) leave
+    both to their corrector (:552-556)."""
     psi = -s.q * o.inv_denom[None]
     psi_n = vmat(o.V, psi)
     q_n = vmat(o.V, s.q)
     w_n = vmat(o.V, vmat(o.w_stencil, s.w))
     th_n = vmat(o.V, vmat(o.S_d, s.th))
     q_adv = q_n if o.eta_hat is None else q_n + o.eta_hat[None]
-    adv = flux23 if o.advection == "flux" else jacobian23
-    if o.advection not in ("flux", "jacobian"):
-        raise ValueError(f"Unsupported nonlinear_advection={o.advection!r}")
-    Aq_n, Aw_n, Ath_n = adv(psi_n, [q_adv, w_n, th_n], o)
+    Aq_n, Aw_n, Ath_n = advection(psi_n, [q_adv, w_n, th_n], o)
     Eq = -vmat(o.V_inv, Aq_n) - 1j * o.beta * o.kx[None] * psi
     Ew = vmat(o.w_pinv, vmat(o.proj_w, -vmat(o.V_inv, Aw_n)))
-    Eth = vmat(o.d_pinv, vmat(o.proj_d, -vmat(o.V_inv, Ath_n)))
+    Etb = np.zeros_like(s.thb)
+    if _closure(o, base) == "evolve_mean" and o.exchange == "legacy":
+        g = o.V @ (o.G_Z @ s.thb)
+        prod = vmat(o.V_inv, g[:, None, None] * w_n)
+        Eth = vmat(o.d_pinv, vmat(o.proj_d, -vmat(o.V_inv, Ath_n) - prod))
+        flux = hmean(w_n, th_n, o, o.Npad)          # horizontal_mean_wtheta (solver.py:204-228)
+        Etb = o.mu * (-(o.G_Z @ (o.V_inv @ flux)))  # mean_flux_exchange_rhs_coeffs (:324-329)
+    else:
+        Eth = vmat(o.d_pinv, vmat(o.proj_d, -vmat(o.V_inv, Ath_n)))
     if o.sponge_phys is not None:
-        Eq = Eq - sponge23(s.q, o)
-        Ew = Ew - sponge23(s.w, o)
-        Eth = Eth - sponge23(s.th, o)
-    return OState(Eq, Ew, Eth, np.zeros_like(s.thb))
+        Eq = Eq - sponge_tendency(s.q, o)
+        Ew = Ew - sponge_tendency(s.w, o)
+        Eth = Eth - sponge_tendency(s.th, o)
+    return OState(Eq, Ew, Eth, Etb)
 
 
-def implicit_tendency(s: OState, o) -> OState:
-    """solver.py:615-641 on the base grid (I_th_bar = 0)."""
+def implicit_tendency(s: OState, o, base: bool = False) -> OState:
+    """solver.py:615-641; I_th_bar = (eps^2/sigma) G_Z2 th_bar under evolve_mean."""
     dw = vmat(o.G_Z, vmat(o.w_stencil, s.w))
     Iw_c = o.inv_denom[None] * vmat(o.G_Z, s.q) + o.Ra_sigma * vmat(o.S_d, s.th)
     Iw = vmat(o.w_pinv, vmat(o.proj_w, Iw_c))
     Ith = s.w if o.map_w_to_th is None else vmat(o.map_w_to_th, s.w)
-    return OState(dw, Iw, Ith, np.zeros_like(s.thb))
+    if _closure(o, base) == "evolve_mean":
+        Itb = (o.mu / o.sigma_mean) * (o.G_Z2 @ s.thb)
+    else:
+        Itb = np.zeros_like(s.thb)
+    return OState(dw, Iw, Ith, Itb)
 
 
 def shell_apply(mats, idx, f):
@@ -446,44 +571,79 @@ def implicit_solve(Rq, Rw, Rth, o):
     return q_new, w_new, th_new
 
 
+def mean_temp_solve(R, o, base: bool = False):
+    """imex_mean_temp_solve (solver.py:741-758): the Dirichlet projection
+    without an evolving mean, else the tau solve of (I - alpha G_Z2)."""
+    if _closure(o, base) != "evolve_mean":
+        return o.proj_d @ R
+    rhs = R.copy()
+    rhs[o.Nz - 1] = 0.0
+    rhs[o.Nz] = 0.0
+    return np.linalg.solve(o.mean_solve_A, rhs)
+
+
 def mean_temp_solve_base(R, o):
-    """imex_mean_temp_solve under fixed_conduction (solver.py:741-744)."""
-    return o.proj_d @ R
+    return mean_temp_solve(R, o, base=True)
 
 
-def sbp_corrector(s: OState, o, total_dt: float) -> OState:
-    """SBP2 thermal corrector with hoisted invariants and n substeps
-    (solver.py:909-950, 873-906, 346-358)."""
-    n = max(1, o.substeps)
-    sdt = total_dt / n
-    mu, kappa = o.mu, 1.0 / o.sigma
-    D1, Lp = o.sbp_D1, o.sbp_L
-    w_sbp = vmat(o.c2s, vmat(o.V, vmat(o.w_stencil, s.w)))
-    w2 = hmean(w_sbp, w_sbp, o)
+def sbp_matrices(w2, sdt, mu, kappa, D1, Lp):
+    """A = I - half, B = I + half with half = mu kappa sdt L / 2 +
+    mu sdt^2 D1 diag<w^2> D1 / 4 and A's boundary rows set to the identity
+    (solver.py:928-942)."""
     half = 0.5 * mu * kappa * sdt * Lp + 0.25 * mu * sdt ** 2 * (D1 @ np.diag(w2) @ D1)
-    I = np.eye(o.Nz + 1)
+    I = np.eye(D1.shape[0])
     A = I - half
     B = I + half
     A[0, :] = 0.0
     A[-1, :] = 0.0
     A[0, 0] = A[-1, -1] = 1.0
-    lu = sla.lu_factor(A)
+    return A, B
+
+
+def sbp_substep(th_sbp, thb_sbp, w_sbp, A, B, sdt, mu, D1, mean):
+    """One SBP-grid substep before the transfer back (solver.py:885-898):
+    implicit mean update from the flux <w theta>, then the fluctuation
+    update from the midpoint gradient of the mean, boundary rows zeroed.
+    ``mean(a, b)`` is the per-level horizontal mean."""
+    flux = mean(w_sbp, th_sbp)
+    rhs = B @ thb_sbp - mu * sdt * (D1 @ flux)
+    rhs[0] = rhs[-1] = 0.0
+    thb_new = np.linalg.solve(A, rhs)
+    gh = 0.5 * (D1 @ (thb_sbp + thb_new))
+    th_new = th_sbp - sdt * w_sbp * gh[:, None, None]
+    th_new[0] = 0.0
+    th_new[-1] = 0.0
+    thb_new[0] = thb_new[-1] = 0.0
+    return th_new, thb_new
+
+
+def sbp_corrector(s: OState, o, total_dt: float) -> OState:
+    """SBP2 thermal corrector with hoisted invariants and n substeps
+    (solver.py:909-950, 873-906, 346-358); identity without an evolving
+    mean (:917-918)."""
+    if o.thermal_closure != "evolve_mean":
+        return s
+    n = max(1, o.substeps)
+    sdt = total_dt / n
+    npad = o.Nx if o.rule == "23_rule" else o.Npad
+    mean = lambda a, b: hmean(a, b, o, npad)
+    w_sbp = vmat(o.c2s, vmat(o.V, vmat(o.w_stencil, s.w)))
+    A, B = sbp_matrices(mean(w_sbp, w_sbp), sdt, o.mu, 1.0 / o.sigma, o.sbp_D1, o.sbp_L)
     th, thb = s.th, s.thb
     for _ in range(n):
         th_sbp = vmat(o.c2s, vmat(o.V, vmat(o.S_d, th)))
         thb_sbp = o.c2s @ (o.V @ thb)
-        flux = hmean(w_sbp, th_sbp, o)
-        rhs = B @ thb_sbp - mu * sdt * (D1 @ flux)
-        rhs[0] = rhs[-1] = 0.0
-        thb_new = sla.lu_solve(lu, rhs)
-        gh = 0.5 * (D1 @ (thb_sbp + thb_new))
-        th_new_sbp = th_sbp - sdt * w_sbp * gh[:, None, None]
-        th_new_sbp[0] = 0.0
-        th_new_sbp[-1] = 0.0
-        thb_new[0] = thb_new[-1] = 0.0
+        th_new_sbp, thb_new = sbp_substep(th_sbp, thb_sbp, w_sbp, A, B, sdt, o.mu,
+                                          o.sbp_D1, mean)
         th = vmat(o.d_pinv, vmat(o.proj_d, vmat(o.V_inv, vmat(o.s2c, th_new_sbp))))
         thb = o.proj_d @ (o.V_inv @ (o.s2c @ thb_new))
     return OState(s.q, s.w, th, thb)
+
+
+def thermal_correction_tendency(p: OState, c: OState, alpha: float) -> OState:
+    """(corrected - predicted) / alpha for theta and th_bar (solver.py:953-962)."""
+    return OState(np.zeros_like(p.q), np.zeros_like(p.w), (c.th - p.th) / alpha,
+                  (c.thb - p.thb) / alpha)
 
 
 def hermitian_project(f, Nx):
@@ -496,10 +656,12 @@ def hermitian_project(f, Nx):
 
 
 def sanitize(s: OState, o) -> OState:
-    """solver.py:107-146 under 23_rule."""
-    m = o.mask[None]
-    return OState(hermitian_project(s.q, o.Nx) * m, hermitian_project(s.w, o.Nx) * m,
-                  hermitian_project(s.th, o.Nx) * m, s.thb)
+    """solver.py:107-146: Hermitian columns, and the 2/3 band under 23_rule."""
+    q, w, th = (hermitian_project(f, o.Nx) for f in (s.q, s.w, s.th))
+    if o.rule == "23_rule":
+        m = o.mask[None]
+        q, w, th = q * m, w * m, th * m
+    return OState(q, w, th, s.thb)
 
 
 def vertical_cutoff(s: OState, o) -> OState:
@@ -523,40 +685,41 @@ def finalize(s: OState, o) -> OState:
 
 
 def step_balanced_sbp2_pc(s: OState, o) -> OState:
-    """imex_step_balanced_sbp2_pc (solver.py:1167-1225)."""
+    """imex_step_balanced_sbp2_pc (solver.py:1167-1225): predictors on the
+    base grid, a corrector after each stage, and stage 2 carrying omega*C1."""
     g = o.gamma_imex
     dt = o.dt
     delta = -np.sqrt(2.0) / 2.0
     alpha = g * dt
     omg = dt * (1.0 - g)
-    E1 = explicit_rhs(s, o)
+    E1 = explicit_rhs(s, o, base=True)
     q1p, w1p, th1p = implicit_solve(s.q + alpha * E1.q, s.w + alpha * E1.w,
                                     s.th + alpha * E1.th, o)
-    p1 = OState(q1p, w1p, th1p, mean_temp_solve_base(s.thb + alpha * E1.thb, o))
+    p1 = OState(q1p, w1p, th1p, mean_temp_solve(s.thb + alpha * E1.thb, o, base=True))
     s1 = sbp_corrector(p1, o, alpha)
-    C1th = (s1.th - p1.th) / alpha
-    C1tb = (s1.thb - p1.thb) / alpha
-    E2 = explicit_rhs(s1, o)
-    I1 = implicit_tendency(s1, o)
+    C1 = thermal_correction_tendency(p1, s1, alpha)
+    E2 = explicit_rhs(s1, o, base=True)
+    I1 = implicit_tendency(s1, o, base=True)
     Rq = s.q + dt * (delta * E1.q + (1 - delta) * E2.q) + omg * I1.q - omg * o.diss_q[None] * s1.q
     Rw = s.w + dt * (delta * E1.w + (1 - delta) * E2.w) + omg * I1.w - omg * o.diss_w[None] * s1.w
     Rth = (s.th + dt * (delta * E1.th + (1 - delta) * E2.th) + omg * I1.th
-           - omg * o.diss_th[None] * s1.th + omg * C1th)
-    Rtb = s.thb + dt * (delta * E1.thb + (1 - delta) * E2.thb) + omg * I1.thb + omg * C1tb
+           - omg * o.diss_th[None] * s1.th + omg * C1.th)
+    Rtb = s.thb + dt * (delta * E1.thb + (1 - delta) * E2.thb) + omg * I1.thb + omg * C1.thb
     q2, w2, th2 = implicit_solve(Rq, Rw, Rth, o)
-    p2 = OState(q2, w2, th2, mean_temp_solve_base(Rtb, o))
+    p2 = OState(q2, w2, th2, mean_temp_solve(Rtb, o, base=True))
     return finalize(sbp_corrector(p2, o, alpha), o)
 
 
-def step_ars222_fixed(s: OState, o) -> OState:
-    """imex_step_ars222 under fixed_conduction (solver.py:1012-1053)."""
+def step_ars222(s: OState, o) -> OState:
+    """imex_step_ars222 (solver.py:1012-1053), fixed_conduction or
+    evolve_mean with the legacy exchange."""
     g = o.gamma_imex
     dt = o.dt
     delta = -np.sqrt(2.0) / 2.0
     E1 = explicit_rhs(s, o)
     q1, w1, th1 = implicit_solve(s.q + g * dt * E1.q, s.w + g * dt * E1.w,
                                  s.th + g * dt * E1.th, o)
-    s1 = OState(q1, w1, th1, mean_temp_solve_base(s.thb + g * dt * E1.thb, o))
+    s1 = OState(q1, w1, th1, mean_temp_solve(s.thb + g * dt * E1.thb, o))
     E2 = explicit_rhs(s1, o)
     I1 = implicit_tendency(s1, o)
     omg = dt * (1 - g)
@@ -565,27 +728,23 @@ def step_ars222_fixed(s: OState, o) -> OState:
     Rth = s.th + dt * (delta * E1.th + (1 - delta) * E2.th) + omg * I1.th - omg * o.diss_th[None] * th1
     Rtb = s.thb + dt * (delta * E1.thb + (1 - delta) * E2.thb) + omg * I1.thb
     q2, w2, th2 = implicit_solve(Rq, Rw, Rth, o)
-    return finalize(OState(q2, w2, th2, mean_temp_solve_base(Rtb, o)), o)
+    return finalize(OState(q2, w2, th2, mean_temp_solve(Rtb, o)), o)
+
+
+step_ars222_fixed = step_ars222
 
 
 def imex_step(s: OState, o) -> OState:
     """imex_step dispatch + sanitize (solver.py:1228-1242)."""
-    if o.thermal_closure == "evolve_mean":
-        if o.exchange != "balanced_sbp2_pc":
-            raise NotImplementedError("oracle covers evolve_mean via balanced_sbp2_pc")
+    if o.exchange == "balanced_sbp2_pc":
         new = step_balanced_sbp2_pc(s, o)
-    elif o.thermal_closure == "fixed_conduction":
-        # With fixed_conduction the SBP corrector is the identity
-        # (solver.py:917-918), so the PC stepper reduces to plain ARS222.
-        new = step_ars222_fixed(s, o)
     else:
-        raise ValueError(f"Unsupported thermal_closure={o.thermal_closure!r}")
+        new = step_ars222(s, o)
     return sanitize(new, o)
 
 
 def full_rhs(s: OState, o) -> OState:
-    """explicit + implicit tendency (solver.py:1249-1258); under
-    fixed_conduction the full-grid explicit RHS is the base-grid one."""
+    """explicit + implicit tendency (solver.py:1249-1258)."""
     E = explicit_rhs(s, o)
     I = implicit_tendency(s, o)
     return OState(E.q + I.q, E.w + I.w, E.th + I.th, E.thb + I.thb)
@@ -600,8 +759,6 @@ def apply_bcs(s: OState, o) -> OState:
 
 def rk4_step(s: OState, o) -> OState:
     """rk4_step, the explicit validator (solver.py:1276-1336)."""
-    if o.thermal_closure != "fixed_conduction":
-        raise NotImplementedError("oracle rk4_step covers fixed_conduction")
     dt = o.dt
 
     def axpy(a, x, k):
@@ -654,8 +811,55 @@ def rfft_weight(Nk):
     return w[None, :]
 
 
+def shell_bins(o):
+    """_shell_bins (diagnostics.py:56-66): |k| shells of width 2 pi / L."""
+    k_mag = np.sqrt(o.ksq)
+    dk = 2.0 * np.pi / o.L
+    n_bins = int(float(np.sqrt(np.max(o.ksq)) / dk)) + 1
+    return k_mag, dk, np.arange(n_bins) * dk + dk / 2
+
+
+def shell_bin_sum(mode_quantity, o):
+    """_shell_bin_sum (diagnostics.py:69-77)."""
+    k_mag, dk, k_bins = shell_bins(o)
+    spec = np.array([np.sum(np.where((k_mag >= i * dk) & (k_mag < (i + 1) * dk),
+                                     mode_quantity, 0.0)) for i in range(k_bins.size)])
+    return k_bins, spec
+
+
+def high_mode_fraction(spec, frac=0.25):
+    """diagnostics.py:590-596."""
+    n_tail = max(1, int(spec.shape[0] * frac))
+    total = np.sum(spec)
+    return float(np.sum(spec[-n_tail:]) / total) if total > 0 else 0.0
+
+
+def sbp_exchange_monitor(s: OState, o) -> dict:
+    """compute_sbp_internal_exchange_budget (diagnostics.py:334-369): the
+    fluctuation-side and mean-side exchange terms under the SBP norm."""
+    keys = ("th_mean_feedback_sum_sbp", "mean_flux_exchange_tendency_sbp",
+            "mean_theta_exchange_boundary_sbp", "mean_theta_exchange_residual_sbp",
+            "mean_theta_exchange_residual_sbp_rel")
+    if o.thermal_closure != "evolve_mean" or o.exchange != "balanced_sbp2_pc":
+        return {k: float("nan") for k in keys}
+    w_sbp = vmat(o.c2s, vmat(o.V, vmat(o.w_stencil, s.w)))
+    th_sbp = vmat(o.c2s, vmat(o.V, vmat(o.S_d, s.th)))
+    thb_sbp = o.c2s @ (o.V @ s.thb)
+    dthb = o.sbp_D1 @ thb_sbp
+    flux = hmean(w_sbp, th_sbp, o, o.Npad)
+    fb = -(dthb @ (o.sbp_H @ flux))
+    ex = -(thb_sbp @ (o.sbp_H @ (o.sbp_D1 @ flux)))
+    bnd = thb_sbp[-1] * flux[-1] - thb_sbp[0] * flux[0]
+    res = fb + ex
+    return dict(zip(keys, (float(fb), float(ex), float(bnd), float(res),
+                           float(res / max(abs(fb) + abs(ex), 1e-300)))))
+
+
 def diagnostics(s: OState, o) -> dict:
-    """Judged subset of compute_diagnostics (diagnostics.py:597-675)."""
+    """compute_diagnostics' scalar set, vertical and horizontal spectra and
+    the SBP exchange monitor (diagnostics.py:597-710).  Nusselt_dealiased
+    uses the solver-consistent flux profile whole (diagnostics.py:206-212);
+    the reference re-sums it from its per-shell parts, equal to roundoff."""
     Nx = o.Nx
     norm = float(Nx) ** 4
     wr = rfft_weight(o.Nk)
@@ -671,25 +875,43 @@ def diagnostics(s: OState, o) -> dict:
     KE_tot = 0.5 * np.sum(o.ksq * psi2 * wr) / norm
     u = _irfft2(-1j * o.ky * psi_n, Nx)
     v = _irfft2(1j * o.kx * psi_n, Nx)
+    w_p = _irfft2(w_n, Nx)
+    th_p = _irfft2(th_n, Nx)
     max_speed = float(np.max(np.sqrt(u * u + v * v)))
     q2 = np.tensordot(cc, np.abs(q_n) ** 2, axes=1)
+    w2 = np.tensordot(cc, np.abs(w_n) ** 2, axes=1)
+    t2 = np.tensordot(cc, np.abs(th_n) ** 2, axes=1)
     enstrophy = 0.5 * np.sum(q2 * wr) / norm
     wth = np.tensordot(cc, np.real(w_n * np.conj(th_n)), axes=1)
-    vol_avg_tw = np.sum(wth * wr) / norm
-    if o.thermal_closure == "evolve_mean" and o.exchange == "balanced_sbp2_pc":
-        w_sbp = vmat(o.c2s, w_n)
-        th_sbp = vmat(o.c2s, th_n)
-        prof = o.s2c @ hmean(w_sbp, th_sbp, o)
-    else:
-        prof = hmean(w_n, th_n, o)
+    vol_avg_tw = float(np.sum(wth * wr) / norm)
+    if o.exchange == "balanced_sbp2_pc":          # sbp2_flux_profile_nodal (solver.py:370-374)
+        prof = o.s2c @ hmean(vmat(o.c2s, w_n), vmat(o.c2s, th_n), o, o.Npad)
+    else:                                          # horizontal_mean_wtheta (solver.py:204-228)
+        prof = hmean(w_n, th_n, o, o.Npad)
     vol_dealiased = float(np.sum(cc * prof))
-    return {
+    q_spec = np.sum(np.abs(s.q) ** 2, axis=(1, 2))
+    w_spec = np.sum(np.abs(w_c) ** 2, axis=(1, 2))
+    th_spec = np.sum(np.abs(th_c) ** 2, axis=(1, 2))
+    k_bins, q_h = shell_bin_sum(q2 * wr, o)
+    _, w_h = shell_bin_sum(w2 * wr, o)
+    _, th_h = shell_bin_sum(t2 * wr, o)
+    out = {
         "KE_tot": float(KE_tot), "KE_bt": float(KE_bt), "KE_bc": float(KE_tot - KE_bt),
-        "max_speed": max_speed, "enstrophy": float(enstrophy),
-        "Nusselt": float(1.0 + vol_avg_tw), "Nusselt_dealiased": 1.0 + vol_dealiased,
-        "vol_avg_tw_dealiased": vol_dealiased,
-        "q_vert_spec": np.sum(np.abs(s.q) ** 2, axis=(1, 2)),
-        "w_vert_spec": np.sum(np.abs(w_c) ** 2, axis=(1, 2)),
-        "th_vert_spec": np.sum(np.abs(th_c) ** 2, axis=(1, 2)),
+        "max_speed": max_speed, "max_w": float(np.max(np.abs(w_p))),
+        "max_theta": float(np.max(np.abs(th_p))), "max_tw": float(np.max(np.abs(w_p * th_p))),
+        "enstrophy": float(enstrophy),
+        "Nusselt": 1.0 + vol_avg_tw, "Nusselt_dealiased": 1.0 + vol_dealiased,
+        "vol_avg_tw": vol_avg_tw, "vol_avg_tw_dealiased": vol_dealiased,
+        "heat_flux_mismatch": vol_dealiased - vol_avg_tw,
+        "flux_profile_dealiased": prof,
+        "q_rms": float(np.sqrt(np.sum(q2 * wr) / norm)),
+        "w_rms": float(np.sqrt(np.sum(w2 * wr) / norm)),
+        "th_rms": float(np.sqrt(np.sum(t2 * wr) / norm)),
+        "q_vert_spec": q_spec, "w_vert_spec": w_spec, "th_vert_spec": th_spec,
+        "q_high_frac": high_mode_fraction(q_spec), "w_high_frac": high_mode_fraction(w_spec),
+        "th_high_frac": high_mode_fraction(th_spec),
+        "k_bins": k_bins, "q_horiz_spec": q_h, "w_horiz_spec": w_h, "th_horiz_spec": th_h,
         "th_bar_max": float(np.max(np.abs(s.thb))),
     }
+    out.update(sbp_exchange_monitor(s, o))
+    return out
