@@ -167,7 +167,7 @@ class Closure:
         return self.out_scale * self.mlp.forward(self.in_scale * np.asarray(x)).astype(np.float64)
 
     def attach(self, grid: Grid) -> None:
-        grid.context(full=False).set_closure(self.mlp, self.n0, self.in_scale, self.out_scale)
+        grid.attach_closure(self.mlp, self.n0, self.in_scale, self.out_scale)
 
     def save(self, path: str, **meta) -> None:
         """FP32 parameters, scales and the top-level offset as .npz (loaded

@@ -24,8 +24,11 @@ def unpack(vec: np.ndarray, nq: int) -> dict:
 
 
 def compute_diagnostics(state, grid) -> dict:
-    from nhqg_amd.solver import Stepper
+    """Diagnostics of a host state (nhqg/diagnostics.py:597), on the grid's
+    scratch context of the layout the state needs."""
+    from nhqg_amd.solver import _as_state, needs_full_layout
 
-    sp = Stepper(grid)
-    sp.set_state(state)
-    return sp.diagnostics()
+    st = _as_state(state)
+    ctx = grid.context(full=needs_full_layout(st, grid))
+    ctx.set_state(*st)
+    return unpack(ctx.diagnostics(), grid.Nz + 1)

@@ -19,6 +19,7 @@ power-of-two ``Nx`` in [16, 1024], ``Nz`` <= 256.
 from __future__ import annotations
 
 import os
+import weakref
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
@@ -309,6 +310,8 @@ class Grid:
     dense_contractions: bool = False
     _layouts: dict = field(default_factory=dict, repr=False)
     _contexts: dict = field(default_factory=dict, repr=False)
+    _live: "weakref.WeakSet" = field(default_factory=weakref.WeakSet, repr=False)
+    _closure: tuple | None = field(default=None, repr=False)
 
     @property
     def evolve_mean(self) -> bool:
@@ -320,14 +323,35 @@ class Grid:
             self._layouts[key] = ModeLayout.build(self, full)
         return self._layouts[key]
 
-    def context(self, full: bool = False):
-        """Device context for this grid (created on first use)."""
+    def new_context(self, full: bool = False):
+        """A device context owned by the caller (each Stepper holds its own,
+        so no other Stepper or per-call function writes its state).  An
+        attached closure is applied to it."""
         from nhqg_amd import _native
 
+        ctx = _native.Context(self, self.layout(full))
+        if self._closure is not None and not full:
+            ctx.set_closure(*self._closure)
+        self._live.add(ctx)
+        return ctx
+
+    def context(self, full: bool = False):
+        """The grid's scratch context of one layout, shared by the per-call
+        functions (imex_step, rk4_step, compute_diagnostics), which set the
+        whole state on every call; created on first use."""
         key = "full" if full else "band"
         if key not in self._contexts:
-            self._contexts[key] = _native.Context(self, self.layout(full))
+            self._contexts[key] = self.new_context(full)
         return self._contexts[key]
+
+    def attach_closure(self, mlp, n0: int, in_scale: float, out_scale: float) -> None:
+        """Attach a trained closure to every band-layout context of this
+        grid, present and future (mlp=None detaches)."""
+        self._closure = None if mlp is None else (mlp, int(n0), float(in_scale),
+                                                  float(out_scale))
+        for ctx in list(self._live):
+            if not ctx.layout.full and getattr(ctx, "h", None):
+                ctx.set_closure(mlp, int(n0), float(in_scale), float(out_scale))
 
     def close(self) -> None:
         for ctx in self._contexts.values():
