@@ -239,6 +239,24 @@ int nhqg_mlp_set_params(nhqg_mlp *m, const float *w);  /* also resets Adam */
 int nhqg_mlp_get_params(nhqg_mlp *m, float *w);
 /* y (batch x n_out) = MLP(x (batch x n_in)) */
 int nhqg_mlp_forward(nhqg_mlp *m, const float *x, int32_t batch, float *y);
+
+/* Allocation-free forward pass.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the call (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * result as nhqg_mlp_forward, but x (batch x n_in) and y (batch x n_out) are
+ * DEVICE pointers owned by the caller, the three layer kernels are enqueued
+ * on `stream` (a hipStream_t; NULL = the null stream) and the call returns
+ * without waiting.  The hidden activations use a fixed device array of
+ * 2 x NHQG_MLP_NOALLOC_MAX_BATCH x NHQG_MLP_NOALLOC_MAX_WIDTH floats (4 MiB per
+ * GPU, kept until the process ends); batch above the maximum, or a network
+ * wider than the maximum, returns NHQG_ERR_ARG with the reason in
+ * nhqg_mlp_last_error.  Every call on one GPU shares that array: calls that
+ * overlap in time on one GPU race.  Nothing in it needs clearing: each call
+ * writes every activation it reads. */
+#define NHQG_MLP_NOALLOC_MAX_BATCH 4096
+#define NHQG_MLP_NOALLOC_MAX_WIDTH 128
+int nhqg_mlp_forward_noalloc(nhqg_mlp *m, const float *x, int32_t batch, float *y,
+                             void *stream);
 /* gradient of mean((MLP(x) - t)^2) w.r.t. the parameters, and the loss */
 int nhqg_mlp_gradient(nhqg_mlp *m, const float *x, const float *t, int32_t batch, float *grad,
                       float *loss);

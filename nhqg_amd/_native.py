@@ -68,7 +68,7 @@ EXPORTS = (
     "nhqg_last_error", "nhqg_stream_copy",
     "nhqg_mlp_create", "nhqg_mlp_destroy", "nhqg_mlp_n_params", "nhqg_mlp_set_params",
     "nhqg_mlp_get_params", "nhqg_mlp_forward", "nhqg_mlp_gradient", "nhqg_mlp_set_data",
-    "nhqg_mlp_train", "nhqg_mlp_last_error", "nhqg_set_closure",
+    "nhqg_mlp_train", "nhqg_mlp_last_error", "nhqg_set_closure", "nhqg_mlp_forward_noalloc",
 )
 
 
@@ -114,6 +114,7 @@ def load():
     lib.nhqg_mlp_set_params.argtypes = [vp, _fp]
     lib.nhqg_mlp_get_params.argtypes = [vp, _fp]
     lib.nhqg_mlp_forward.argtypes = [vp, _fp, C.c_int32, _fp]
+    lib.nhqg_mlp_forward_noalloc.argtypes = [vp, vp, C.c_int32, vp, vp]
     lib.nhqg_mlp_gradient.argtypes = [vp, _fp, _fp, C.c_int32, _fp, _fp]
     lib.nhqg_mlp_set_data.argtypes = [vp, _fp, _fp, C.c_int32]
     lib.nhqg_mlp_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, _fp]
@@ -350,6 +351,13 @@ class MLP:
         self._check(self.lib.nhqg_mlp_forward(self.h, _ptr(x, fp), x.shape[0], _ptr(y, fp)),
                     "nhqg_mlp_forward")
         return y
+
+    def forward_noalloc(self, x_dev: int, batch: int, y_dev: int, stream: int = 0) -> None:
+        """Allocation-free forward pass on device pointers (x: batch x n_in,
+        y: batch x n_out float32), enqueued on `stream` without waiting."""
+        self._check(self.lib.nhqg_mlp_forward_noalloc(self.h, C.c_void_p(x_dev), int(batch),
+                                                      C.c_void_p(y_dev), C.c_void_p(stream)),
+                    "nhqg_mlp_forward_noalloc")
 
     def gradient(self, x, t):
         x, t = self._f(x), self._f(t)

@@ -15,6 +15,7 @@
 // imaginary parts) get Hermitian tendencies.  Oddness is also what
 // statistical translation invariance asks of a per-column predictor: a
 // shift by half a wavelength maps a column to its negative.
+#include <mutex>
 #include <vector>
 #include <string>
 #include <cstring>
@@ -183,16 +184,22 @@ void zero_bias(nhqg_mlp* m, float* p) {
     }                                                                         \
   } while (0)
 
-int forward(nhqg_mlp* m, int batch) {
+// The three layers on stream st: x -> a1 -> a2 -> y (all device memory).
+int forward_into(nhqg_mlp* m, int batch, const float* x, float* a1, float* a2, float* y,
+                 hipStream_t st) {
   const float* w = m->w;
-  hipLaunchKernelGGL(mlp_dense_fwd, dim3(nblk((size_t)batch * m->width)), dim3(256), 0, m->st,
-                     batch, m->n_in, m->width, w, w + off_b1(m), m->x, m->a1, 1);
-  hipLaunchKernelGGL(mlp_dense_fwd, dim3(nblk((size_t)batch * m->width)), dim3(256), 0, m->st,
-                     batch, m->width, m->width, w + off_w2(m), w + off_b2(m), m->a1, m->a2, 1);
-  hipLaunchKernelGGL(mlp_dense_fwd, dim3(nblk((size_t)batch * m->n_out)), dim3(256), 0, m->st,
-                     batch, m->width, m->n_out, w + off_w3(m), w + off_b3(m), m->a2, m->y, 0);
+  hipLaunchKernelGGL(mlp_dense_fwd, dim3(nblk((size_t)batch * m->width)), dim3(256), 0, st,
+                     batch, m->n_in, m->width, w, w + off_b1(m), x, a1, 1);
+  hipLaunchKernelGGL(mlp_dense_fwd, dim3(nblk((size_t)batch * m->width)), dim3(256), 0, st,
+                     batch, m->width, m->width, w + off_w2(m), w + off_b2(m), a1, a2, 1);
+  hipLaunchKernelGGL(mlp_dense_fwd, dim3(nblk((size_t)batch * m->n_out)), dim3(256), 0, st,
+                     batch, m->width, m->n_out, w + off_w3(m), w + off_b3(m), a2, y, 0);
   MK(hipGetLastError());
   return NHQG_OK;
+}
+
+int forward(nhqg_mlp* m, int batch) {
+  return forward_into(m, batch, m->x, m->a1, m->a2, m->y, m->st);
 }
 
 int backward(nhqg_mlp* m, int batch, float* hist = nullptr) {
@@ -565,3 +572,64 @@ int nhqg_mlp_train(nhqg_mlp* m, int32_t n_steps, int32_t batch, int32_t start, f
 const char* nhqg_mlp_last_error(const nhqg_mlp* m) { return m ? m->err.c_str() : ""; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free forward pass.  The two hidden activations live in one
+// file-scope device array (4 MiB: NHQG_MLP_NOALLOC_MAX_BATCH rows of up to
+// NHQG_MLP_NOALLOC_MAX_WIDTH floats, twice), held by every GPU the library
+// runs on until the process ends.  Host code never passes the array's name
+// to a kernel: its device address, taken once per device with that device
+// current by hipGetSymbolAddress, is kept in g_scratch_ptr[device].
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) float
+    scratch[2 * NHQG_MLP_NOALLOC_MAX_BATCH * NHQG_MLP_NOALLOC_MAX_WIDTH];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(float** out, std::string& err) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    err = "nhqg_mlp_forward_noalloc: no usable current device";
+    return NHQG_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+      return NHQG_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return NHQG_OK;
+}
+}  // namespace
+
+int nhqg_mlp_forward_noalloc(nhqg_mlp* m, const float* x, int32_t batch, float* y,
+                             void* stream) {
+  if (!m || !x || !y) return NHQG_ERR_ARG;
+  if (batch < 1 || batch > NHQG_MLP_NOALLOC_MAX_BATCH) {
+    m->err = "nhqg_mlp_forward_noalloc: batch must be in [1, " +
+             std::to_string(NHQG_MLP_NOALLOC_MAX_BATCH) + "]";
+    return NHQG_ERR_ARG;
+  }
+  if (m->width > NHQG_MLP_NOALLOC_MAX_WIDTH) {
+    m->err = "nhqg_mlp_forward_noalloc: width must be at most " +
+             std::to_string(NHQG_MLP_NOALLOC_MAX_WIDTH);
+    return NHQG_ERR_ARG;
+  }
+  float* s = nullptr;
+  const int r = scratch_for_current_device(&s, m->err);
+  if (r) return r;
+  // every element of both activations is written before it is read: nothing
+  // in the scratch array needs clearing between calls
+  float* a1 = s;
+  float* a2 = s + (size_t)NHQG_MLP_NOALLOC_MAX_BATCH * NHQG_MLP_NOALLOC_MAX_WIDTH;
+  return forward_into(m, batch, x, a1, a2, y, static_cast<hipStream_t>(stream));
+}
