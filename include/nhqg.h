@@ -47,7 +47,7 @@ typedef struct {
   int32_t n_shells;    /* number of |k|^2 shells in shell_inv */
   int32_t n_pairs;     /* entries of herm_pairs */
   int32_t substeps;    /* SBP corrector substeps (sbp_corrector_substeps) */
-  int32_t evolve_mean; /* 1: evolve_mean + balanced_sbp2_pc corrector */
+  int32_t evolve_mean; /* 1: thermal_closure = evolve_mean (th_bar evolves; see exchange) */
   int32_t advection;   /* 0: flux form (the production branch), 1: Jacobian form
                         * (nonlinear_advection = "jacobian", nhqg/spectral.py:281-304) */
   int32_t w_neumann;   /* 1: w in the mixed bottom-Dirichlet/top-Neumann basis */
@@ -64,6 +64,11 @@ typedef struct {
                                * of half size, half the flops of the dense product, same
                                * result to roundoff; 1: every operator dense (no reference
                                * counterpart: the reference applies dense matrices) */
+  int32_t exchange;    /* mean_exchange_discretization: 0 balanced_sbp2_pc (the SBP corrector
+                        * under evolve_mean; the diagnostics' heat-flux profile on the SBP
+                        * grid, diagnostics.py:206-208), 1 legacy (under evolve_mean the
+                        * -dTheta/dZ w and -eps^2 d<w theta>/dZ terms of explicit_rhs,
+                        * solver.py:575-590, and the mean's tau solve, solver.py:746-758) */
   double dt, gamma_imex, ra_sigma, sigma, eps_sq, beta;
 } nhqg_params;
 
@@ -132,6 +137,19 @@ typedef struct {
   /* Jacobian form with a trap: d(eta)/dx, d(eta)/dy on the grid (Nx x Nx,
    * row x, column y), the gradient J(psi, q + eta) adds to q's */
   const double *eta_x_phys, *eta_y_phys;
+  /* mean temperature, Nq x Nq: VGZ = V G_Z (dTheta/dZ at the CGL levels);
+   * mean_solve = inverse of (I - gamma dt eps^2/sigma G_Z2) with its last two
+   * rows replaced by the Dirichlet tau rows, applied to a right-hand side
+   * whose last two entries are dropped (imex_mean_temp_solve,
+   * solver.py:746-758); mean_diff = eps^2/sigma G_Z2 (I_th_bar, :638);
+   * mean_flux = -eps^2 G_Z V_inv (E_th_bar from the CGL flux profile,
+   * :585-590, 324-329) */
+  const double *VGZ, *mean_solve, *mean_diff, *mean_flux;
+  /* per-mode weights and (-kx) partners of the horizontal mean on the
+   * zero-padded 3/2 grid (horizontal_mean_from_nodal_spectral with Npad,
+   * solver.py:231-245): equal to pweight/partner on band-limited layouts */
+  const double *pweight_pad;
+  const int32_t *partner_pad;
 } nhqg_operators;
 
 /* Diagnostics vector layout returned by nhqg_diagnostics (length
@@ -148,7 +166,22 @@ typedef struct {
 #define NHQG_DIAG_Q_RMS 9
 #define NHQG_DIAG_W_RMS 10
 #define NHQG_DIAG_TH_RMS 11
-#define NHQG_DIAG_SCALARS 12
+#define NHQG_DIAG_MAX_W 12           /* max |w| on the grid at the CGL levels */
+#define NHQG_DIAG_MAX_THETA 13       /* max |theta| */
+#define NHQG_DIAG_MAX_TW 14          /* max |w theta| */
+#define NHQG_DIAG_VOLTW 15           /* vol_avg_tw (CGL, Parseval) */
+#define NHQG_DIAG_HEAT_FLUX_MISMATCH 16 /* vol_avg_tw_dealiased - vol_avg_tw */
+#define NHQG_DIAG_Q_HIGH_FRAC 17     /* energy fraction of the top quarter of */
+#define NHQG_DIAG_W_HIGH_FRAC 18     /* the vertical spectrum (diagnostics.py:590-596) */
+#define NHQG_DIAG_TH_HIGH_FRAC 19
+/* SBP-grid exchange monitor (diagnostics.py:334-369; NaN unless evolve_mean
+ * with the balanced_sbp2_pc exchange, and then not counted as non-finite) */
+#define NHQG_DIAG_MON_FEEDBACK 20    /* th_mean_feedback_sum_sbp */
+#define NHQG_DIAG_MON_EXCHANGE 21    /* mean_flux_exchange_tendency_sbp */
+#define NHQG_DIAG_MON_BOUNDARY 22    /* mean_theta_exchange_boundary_sbp */
+#define NHQG_DIAG_MON_RESIDUAL 23    /* mean_theta_exchange_residual_sbp */
+#define NHQG_DIAG_MON_RESIDUAL_REL 24 /* mean_theta_exchange_residual_sbp_rel */
+#define NHQG_DIAG_SCALARS 25
 
 /* Build a context: uploads operators, allocates the device state and
  * workspace, captures the step graph lazily.  Replaces make_grid's device
@@ -177,6 +210,13 @@ int nhqg_sync(nhqg_ctx *ctx);
  * (scripts/run_polar.py:364-367). */
 int nhqg_diagnostics(nhqg_ctx *ctx, double *out, int32_t n_out);
 
+/* Depth-integrated power of each stored mode of the nodal q, w, theta,
+ * sum_j cc_j |f_j(k)|^2: out[f * nm + m] for f = q, w, theta and the mode
+ * layout's modes m (3 * nm values).  Binned over |k| shells with the rfft
+ * weights it gives compute_diagnostics' q/w/th_horiz_spec
+ * (nhqg/diagnostics.py:99-108, 683-685). */
+int nhqg_mode_power(nhqg_ctx *ctx, double *out, int32_t n_out);
+
 /* Component entry points on the device state, for parity testing:
  *   NHQG_OP_EXPLICIT_RHS    aux <- explicit_rhs(state)    solver.py:510-599
  *   NHQG_OP_IMPLICIT_TEND   aux <- implicit_tendency(state) solver.py:615-641
@@ -186,8 +226,8 @@ int nhqg_diagnostics(nhqg_ctx *ctx, double *out, int32_t n_out);
  *                                                         solver.py:909-950
  *   NHQG_OP_SANITIZE        state <- finalize + sanitize  solver.py:128-146
  *   NHQG_OP_RK4_STEP        state <- rk4_step(state), the explicit RK4
- *                           validator (fixed_conduction configs; evolve_mean
- *                           returns NHQG_ERR_UNSUPPORTED)  solver.py:1276-1336
+ *                           validator, including th_bar's tendency under
+ *                           evolve_mean (either exchange)  solver.py:1276-1336
  * nhqg_get_aux copies the aux fields (same layout as the state, th_bar
  * part returned as zeros). */
 #define NHQG_OP_EXPLICIT_RHS 1

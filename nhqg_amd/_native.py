@@ -26,7 +26,10 @@ OP_RK4_STEP = 6
 MLP_ODD = 1
 DIAG_NAMES = ("KE_tot", "KE_bt", "KE_bc", "max_speed", "enstrophy", "Nusselt",
               "Nusselt_dealiased", "vol_avg_tw_dealiased", "th_bar_max", "q_rms", "w_rms",
-              "th_rms")
+              "th_rms", "max_w", "max_theta", "max_tw", "vol_avg_tw", "heat_flux_mismatch",
+              "q_high_frac", "w_high_frac", "th_high_frac", "th_mean_feedback_sum_sbp",
+              "mean_flux_exchange_tendency_sbp", "mean_theta_exchange_boundary_sbp",
+              "mean_theta_exchange_residual_sbp", "mean_theta_exchange_residual_sbp_rel")
 N_DIAG_SCALARS = len(DIAG_NAMES)
 
 _dp = C.POINTER(C.c_double)
@@ -36,7 +39,8 @@ _ip = C.POINTER(C.c_int32)
 class Params(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "nx", "nz", "nkx", "nky", "n_shells", "n_pairs", "substeps", "evolve_mean",
-        "advection", "w_neumann", "apply_mask", "mixed_slices", "dense_contractions")] + [
+        "advection", "w_neumann", "apply_mask", "mixed_slices", "dense_contractions",
+        "exchange")] + [
         (n, C.c_double) for n in ("dt", "gamma_imex", "ra_sigma", "sigma", "eps_sq", "beta")]
 
 
@@ -53,6 +57,8 @@ _OPS_FIELDS = [
     ("eta_phys", _dp), ("sponge_phys", _dp), ("twiddle", _dp), ("shell_B0", _dp),
     ("shell_K", _dp), ("shell_coef", _dp), ("eig_Q", _dp), ("eig_Qinv", _dp), ("eig_lam", _dp),
     ("mode_alpha", _dp), ("mode_c", _dp), ("eta_x_phys", _dp), ("eta_y_phys", _dp),
+    ("VGZ", _dp), ("mean_solve", _dp), ("mean_diff", _dp), ("mean_flux", _dp),
+    ("pweight_pad", _dp), ("partner_pad", _ip),
 ]
 
 
@@ -69,6 +75,7 @@ EXPORTS = (
     "nhqg_mlp_create", "nhqg_mlp_destroy", "nhqg_mlp_n_params", "nhqg_mlp_set_params",
     "nhqg_mlp_get_params", "nhqg_mlp_forward", "nhqg_mlp_gradient", "nhqg_mlp_set_data",
     "nhqg_mlp_train", "nhqg_mlp_last_error", "nhqg_set_closure", "nhqg_mlp_forward_noalloc",
+    "nhqg_mode_power",
 )
 
 
@@ -98,6 +105,7 @@ def load():
     lib.nhqg_step_async.argtypes = [vp, C.c_int32]
     lib.nhqg_sync.argtypes = [vp]
     lib.nhqg_diagnostics.argtypes = [vp, _dp, C.c_int32]
+    lib.nhqg_mode_power.argtypes = [vp, _dp, C.c_int32]
     lib.nhqg_apply.argtypes = [vp, C.c_int32, C.c_double]
     lib.nhqg_get_aux.argtypes = [vp, _dp, _dp, _dp]
     lib.nhqg_last_step_ms.argtypes = [vp, _dp]
@@ -173,6 +181,7 @@ class Context:
         p.apply_mask = 1 if layout.full else 0
         p.mixed_slices = int(getattr(grid, "mixed_slices", 0))
         p.dense_contractions = int(getattr(grid, "dense_contractions", 0))
+        p.exchange = 0 if grid.mean_exchange_discretization == "balanced_sbp2_pc" else 1
         p.dt, p.gamma_imex = grid.dt, grid.gamma_imex
         p.ra_sigma, p.sigma = grid.Ra_sigma, grid.sigma
         p.eps_sq, p.beta = grid.mean_temp_eps_sq, grid.beta
@@ -263,6 +272,13 @@ class Context:
         if rc != NHQG_NONFINITE:
             self._check(rc, "nhqg_diagnostics")
         return out
+
+    def mode_power(self):
+        """(3, nm): sum_j cc_j |f_j|^2 of the nodal q, w, theta per mode."""
+        nm = self.layout.nm
+        out = np.zeros(3 * nm, np.float64)
+        self._check(self.lib.nhqg_mode_power(self.h, _ptr(out), 3 * nm), "nhqg_mode_power")
+        return out.reshape(3, nm)
 
     def profile_step(self):
         """Advance one step eagerly with per-kernel-class HIP event timing.

@@ -5,6 +5,7 @@
 // (nhqg/solver.py:1167-1225, dispatched by imex_step :1228-1242) captured
 // once into a hipGraph and replayed per step.
 #include <string>
+#include <limits>
 #include <vector>
 #include <cstring>
 #include <cstdio>
@@ -34,6 +35,11 @@ struct Ops {
       *diss_th, *mask, *pw, *ones, *pw_ke, *pw_cc;
   double *shell_inv, *eta_phys, *sponge_phys, *eta_x, *eta_y;
   double2* tw;
+  // mean temperature (evolve_mean): V G_Z, tau solve, diffusion, flux map
+  double *vgz, *msolve, *mdiff, *mflux;
+  // horizontal-mean weights and partners of the padded 3/2 grid
+  double* pw_pad;
+  int* partner_pad;
 };
 
 }  // namespace
@@ -73,6 +79,12 @@ struct nhqg_ctx {
   size_t fstride;
   double2* staging;
   double *diag, *rowmax;
+  // mean-temperature vectors (Nq): dTheta/dZ at the CGL levels, E_th_bar of
+  // the two stages, I_th_bar, and the RK4 validator's th_bar stage values
+  double *mgrad = nullptr, *etb1 = nullptr, *etb2 = nullptr, *itb = nullptr, *ktb = nullptr,
+         *acctb = nullptr, *stb = nullptr;
+  double* mpow = nullptr;            // nhqg_mode_power: 3 x nm
+  std::vector<double> h_thb2s, h_d1b;  // host copies for the exchange monitor
   // per-launch timing of one eager step (nhqg_profile_step)
   struct PRec {
     int cls;
@@ -361,12 +373,31 @@ int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
 // (the stage-1 predictor right-hand sides) from the same accumulators, for
 // each field whose E is final there (E_q only without beta and closure);
 // the return value flags the fields done that way (bit f).
+//
+// Under evolve_mean with the legacy exchange (thb and Etb given) it also
+// forms E_th_bar = -eps^2 G_Z V_inv <w theta> from the padded-grid means of
+// the nodal w, theta, and adds dTheta/dZ * w (nodal) to the advection of
+// theta before its back transform: E_th = -P_d V_inv (A_th + dTheta/dZ w)
+// (nhqg/solver.py:575-590).
 int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, double* Eq,
                  double* Ew, double* Eth, double* const* R = nullptr, double al = 0.0,
-                 int* fused = nullptr, const Epi* th_tail = nullptr, double th_scale = 0.0) {
+                 int* fused = nullptr, const Epi* th_tail = nullptr, double th_scale = 0.0,
+                 const double* thb = nullptr, double* Etb = nullptr) {
   const Ops& o = c->o;
   RUN(gemm_jobs(c, {Job{&o.V, q, c->qn, epi_make(1.0)}, Job{&o.VSw, w, c->wn, epi_make(1.0)},
                     Job{&o.VSd, th, c->thn, epi_make(1.0)}}));
+  const bool legacy = thb && Etb;
+  if (legacy) {
+    const int n = c->Nq;
+    PL(CL_PARSEVAL, 32.0 * n * (double)c->nm,
+       launch_parseval(n, c->nm, c->ld, c->Pred, c->wn, c->thn, o.pw_pad, o.partner_pad,
+                       c->partial, c->st));
+    PL(CL_SMALL, 0.0, launch_partial_matvec(n, n, c->Pred, c->partial, o.mflux, Etb, c->st));
+    PL(CL_SMALL, 0.0, launch_small_matvec(n, n, o.vgz, thb, c->mgrad, c->st));
+    // dTheta/dZ * w at the CGL levels, held until the advection is done
+    PL(CL_ELEMENTWISE, 32.0 * n * (double)c->nm,
+       launch_level_scale(n, c->ld, c->mgrad, c->wn, c->thsbp2, c->st));
+  }
   FluxArgs a;
   a.N = c->Nx;
   a.nq = c->Nq;
@@ -409,6 +440,12 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   PL(CL_FLUX_COLINV, (3 + sp) * Fc + (nxin + sp) * F, launch_flux(a, 1, c->st));
   PL(CL_FLUX_ROW, (nxin + nyout + 2 * sp) * F, launch_flux(a, 2, c->st));
   PL(CL_FLUX_COLFWD, (nyout + sp) * F + (3 + sp) * Fc, launch_flux(a, 3, c->st));
+  if (legacy) {
+    Epi e = epi_make(0.0);
+    epi_add(e, 1.0, c->thn);
+    epi_add(e, 1.0, c->thsbp2);
+    RUN(combine(c, c->Nq, c->thn, e));
+  }
   // E = -(back transform of the advection) - sponge (solver.py:541-549, 595-597)
   Epi eq = epi_make(-1.0), ew = epi_make(-1.0), et = epi_make(-1.0);
   if (a.sponge_phys) {
@@ -646,13 +683,19 @@ int enqueue_step_body(nhqg_ctx* c) {
   const double om = dt * (1.0 - g);
   const double delta = -0.70710678118654752440;  // -sqrt(2)/2
   const int Nq = c->Nq, Ng = c->Ng;
-  const bool em = c->P.evolve_mean != 0;
+  // evolve_mean: balanced_sbp2_pc (predictors on the base grid, an SBP
+  // corrector after each stage, solver.py:1167-1225) or the legacy exchange
+  // (imex_step_ars222 with the exchange in the explicit RHS and the mean's
+  // tau solve, solver.py:1012-1053)
+  const bool legacy = c->P.evolve_mean != 0 && c->P.exchange == 1;
+  const bool em = c->P.evolve_mean != 0 && !legacy;
 
   // stage 1 predictor; R = y + al*E1 comes out of the back transforms
   // where E1 is final there
   double* const R1[3] = {c->Rq, c->Rw, c->Rth};
   int fused = 0;
-  RUN(explicit_rhs(c, c->yq, c->yw, c->yth, c->E1q, c->E1w, c->E1th, R1, al, &fused));
+  RUN(explicit_rhs(c, c->yq, c->yw, c->yth, c->E1q, c->E1w, c->E1th, R1, al, &fused, nullptr,
+                   0.0, legacy ? c->ythb : nullptr, legacy ? c->etb1 : nullptr));
   {
     const double* ys[3] = {c->yq, c->yw, c->yth};
     const double* es[3] = {c->E1q, c->E1w, c->E1th};
@@ -670,7 +713,14 @@ int enqueue_step_body(nhqg_ctx* c) {
   RUN(implicit_solve_wth(c, c->Rq, c->Rw, c->Rth, c->s1w, c->s1th));
   if (em) RUN(corrector_preamble_fork(c, c->s1w, al, &fk));
   RUN(implicit_solve_q(c, c->Rq, c->s1w, c->s1q));
-  PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->ythb, c->s1thb, c->st));
+  if (legacy) {
+    // th_bar1 = mean_solve (th_bar + al E1_th_bar)
+    PL(CL_SMALL, 0.0, launch_small_lincomb(Nq, c->Rthb, 1.0, c->ythb, al, c->etb1, 0.0, nullptr,
+                                           0.0, nullptr, c->st));
+    PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.msolve, c->Rthb, c->s1thb, c->st));
+  } else {
+    PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->ythb, c->s1thb, c->st));
+  }
   // stage 1 corrector
   const double* th1 = c->s1th;
   const double* thb1 = c->s1thb;
@@ -694,7 +744,8 @@ int enqueue_step_body(nhqg_ctx* c) {
     }
   }
   RUN(explicit_rhs(c, c->s1q, c->s1w, th1, c->E2q, c->E2w, rth_fused ? c->Rth : c->E2th, nullptr,
-                   0.0, nullptr, rth_fused ? &htail : nullptr, dt * (1.0 - delta)));
+                   0.0, nullptr, rth_fused ? &htail : nullptr, dt * (1.0 - delta),
+                   legacy ? c->s1thb : nullptr, legacy ? c->etb2 : nullptr));
   {
     // R_q2 = y + dt(delta E1 + (1-delta) E2) + om*(G_Z S_w w1) - om*diss_q*q1
     Epi e = epi_make(om);
@@ -735,26 +786,36 @@ int enqueue_step_body(nhqg_ctx* c) {
       }
       RUN(combine(c, Ng, c->Rth, h));
     }
-    if (em)
+    if (em) {
       PL(CL_SMALL, 0.0, launch_small_axpy(Nq, c->ythb, thb1, c->s1thb, om / al, c->Rthb, c->st));
-    else
+    } else if (legacy) {
+      // R_th_bar2 = th_bar + dt (delta E1 + (1-delta) E2) + om eps^2/sigma G_Z2 th_bar1
+      PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.mdiff, c->s1thb, c->itb, c->st));
+      PL(CL_SMALL, 0.0, launch_small_lincomb(Nq, c->Rthb, 1.0, c->ythb, dt * delta, c->etb1,
+                                             dt * (1.0 - delta), c->etb2, om, c->itb, c->st));
+    } else {
       PL(CL_SMALL, 0.0, launch_small_axpy(Nq, c->ythb, c->ythb, c->ythb, 0.0, c->Rthb, c->st));
+    }
   }
   RUN(implicit_solve_wth(c, c->Rq, c->Rw, c->Rth, c->yw, c->yth));
   if (em) RUN(corrector_preamble_fork(c, c->yw, al, &fk));
   RUN(implicit_solve_q(c, c->Rq, c->yw, c->yq));
-  PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->ythb, c->st));
+  PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, legacy ? o.msolve : o.proj_d, c->Rthb, c->ythb,
+                                        c->st));
   if (em) RUN(corrector(c, c->yw, c->yth, c->ythb, c->yth, c->ythb, al, fk));
   RUN(sanitize(c));
   return NHQG_OK;
 }
 
 // ---------------------------------------------------------------------------
-// RK4 validator (nhqg/solver.py:1249-1336), fixed_conduction configs: the
-// full RHS is the explicit RHS plus the implicit tendency (E_th_bar and
-// I_th_bar are zero there).  The stage states' _apply_bcs is the identity on
-// Galerkin-basis fields (pinv * proj * S = I) and is skipped; th_bar gets its
-// projection at the end as in the reference.
+// RK4 validator (nhqg/solver.py:1249-1336): the full RHS is the explicit RHS
+// plus the implicit tendency.  Under evolve_mean the th_bar tendency is
+// I_th_bar = eps^2/sigma G_Z2 th_bar, plus E_th_bar and theta's
+// -dTheta/dZ w term with the legacy exchange (split exchanges leave both to
+// their substep, solver.py:552-556).  The stage states' _apply_bcs is the
+// identity on Galerkin-basis fields (pinv * proj * S = I) and is skipped for
+// them; th_bar is projected at every stage and at the end, as in the
+// reference.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 exp_scale_kernel(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
@@ -763,10 +824,22 @@ exp_scale_kernel(int n, double a, const double* __restrict__ x, double* __restri
 }
 
 // K <- explicit_rhs(s) + implicit_tendency(s) (solver.py:1249-1258)
-int full_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, double* Kq,
-             double* Kw, double* Kth) {
+int full_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, const double* thb,
+             double* Kq, double* Kw, double* Kth, double* Ktb) {
   const Ops& o = c->o;
-  RUN(explicit_rhs(c, q, w, th, Kq, Kw, Kth));
+  const bool em = c->P.evolve_mean != 0;
+  const bool legacy = em && c->P.exchange == 1;
+  RUN(explicit_rhs(c, q, w, th, Kq, Kw, Kth, nullptr, 0.0, nullptr, nullptr, 0.0,
+                   legacy ? thb : nullptr, legacy ? Ktb : nullptr));
+  // th_bar: (E_th_bar) + eps^2/sigma G_Z2 th_bar, or zero
+  if (em) {
+    PL(CL_SMALL, 0.0, launch_small_matvec(c->Nq, c->Nq, o.mdiff, thb, c->itb, c->st));
+    PL(CL_SMALL, 0.0, launch_small_lincomb(c->Nq, Ktb, legacy ? 1.0 : 0.0, legacy ? Ktb : nullptr,
+                                           1.0, c->itb, 0.0, nullptr, 0.0, nullptr, c->st));
+  } else {
+    PL(CL_SMALL, 0.0, launch_small_lincomb(c->Nq, Ktb, 0.0, nullptr, 0.0, nullptr, 0.0, nullptr,
+                                           0.0, nullptr, c->st));
+  }
   // I_q = G_Z S_w w, added to E_q in the contraction's epilogue (in place)
   Epi e = epi_make(1.0);
   epi_add(e, 1.0, Kq);
@@ -797,14 +870,14 @@ int full_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, do
 }
 
 int rk4_step(nhqg_ctx* c) {
-  if (c->P.evolve_mean) {
-    c->err = "rk4_step covers thermal_closure='fixed_conduction' (the evolve_mean RHS with the "
-             "mean-temperature coupling is not built)";
-    return NHQG_ERR_UNSUPPORTED;
-  }
   const Ops& o = c->o;
   const double dt = c->P.dt;
   const int Nq = c->Nq, Ng = c->Ng;
+  if (c->P.evolve_mean && (!o.mdiff || (c->P.exchange == 1 && (!o.mflux || !o.vgz)))) {
+    c->err = "rk4_step under evolve_mean needs the mean_diff (and, for the legacy exchange, "
+             "mean_flux and VGZ) operators";
+    return NHQG_ERR_ARG;
+  }
   if (!c->rk4_dq) {
     RUN(dalloc(c, &c->rk4_dq, (size_t)c->nm_pad));
     RUN(dalloc(c, &c->rk4_dw, (size_t)c->nm_pad));
@@ -823,7 +896,7 @@ int rk4_step(nhqg_ctx* c) {
   double* k[3] = {c->E1q, c->E1w, c->E1th};
   double* acc[3] = {c->Rq, c->Rw, c->Rth};
   const int nl[3] = {Nq, Ng, Ng};
-  // out <- y + a * k (+ b * acc)
+  // stage state s = y + a k (th_bar projected); with_acc: acc += a k
   auto axpy = [&](double* const* out, double a, bool with_acc) -> int {
     for (int f = 0; f < 3; ++f) {
       Epi e = epi_make(0.0);
@@ -831,26 +904,36 @@ int rk4_step(nhqg_ctx* c) {
       epi_add(e, a, k[f]);
       RUN(combine(c, nl[f], out[f], e));
     }
+    if (with_acc) {
+      PL(CL_SMALL, 0.0, launch_small_lincomb(Nq, c->acctb, 1.0, c->acctb, a, c->ktb, 0.0, nullptr,
+                                             0.0, nullptr, c->st));
+    } else {
+      PL(CL_SMALL, 0.0, launch_small_lincomb(Nq, c->Rthb, 1.0, c->ythb, a, c->ktb, 0.0, nullptr,
+                                             0.0, nullptr, c->st));
+      PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->stb, c->st));
+    }
     return NHQG_OK;
   };
-  RUN(full_rhs(c, y[0], y[1], y[2], k[0], k[1], k[2]));          // k1
+  RUN(full_rhs(c, y[0], y[1], y[2], c->ythb, k[0], k[1], k[2], c->ktb));   // k1
   {
-    // acc = y + dt/6 k1 (acc's input is y here)
+    // acc = y + dt/6 k1
     for (int f = 0; f < 3; ++f) {
       Epi e = epi_make(0.0);
       epi_add(e, 1.0, y[f]);
       epi_add(e, dt / 6.0, k[f]);
       RUN(combine(c, nl[f], acc[f], e));
     }
+    PL(CL_SMALL, 0.0, launch_small_lincomb(Nq, c->acctb, 1.0, c->ythb, dt / 6.0, c->ktb, 0.0,
+                                           nullptr, 0.0, nullptr, c->st));
   }
-  RUN(axpy(sv, 0.5 * dt, false));                                  // s2 = y + dt/2 k1
-  RUN(full_rhs(c, sv[0], sv[1], sv[2], k[0], k[1], k[2]));        // k2
-  RUN(axpy(acc, dt / 3.0, true));                                  // acc += dt/3 k2
-  RUN(axpy(sv, 0.5 * dt, false));                                  // s3 = y + dt/2 k2
-  RUN(full_rhs(c, sv[0], sv[1], sv[2], k[0], k[1], k[2]));        // k3
-  RUN(axpy(acc, dt / 3.0, true));                                  // acc += dt/3 k3
-  RUN(axpy(sv, dt, false));                                        // s4 = y + dt k3
-  RUN(full_rhs(c, sv[0], sv[1], sv[2], k[0], k[1], k[2]));        // k4
+  RUN(axpy(sv, 0.5 * dt, false));                                             // s2
+  RUN(full_rhs(c, sv[0], sv[1], sv[2], c->stb, k[0], k[1], k[2], c->ktb));   // k2
+  RUN(axpy(acc, dt / 3.0, true));
+  RUN(axpy(sv, 0.5 * dt, false));                                             // s3
+  RUN(full_rhs(c, sv[0], sv[1], sv[2], c->stb, k[0], k[1], k[2], c->ktb));   // k3
+  RUN(axpy(acc, dt / 3.0, true));
+  RUN(axpy(sv, dt, false));                                                   // s4
+  RUN(full_rhs(c, sv[0], sv[1], sv[2], c->stb, k[0], k[1], k[2], c->ktb));   // k4
   // y = (acc + dt/6 k4) * exp(-rate dt), the RK4-only post-step dissipation
   const double* dz[3] = {c->rk4_dq, c->rk4_dw, c->rk4_dth};
   for (int f = 0; f < 3; ++f) {
@@ -860,9 +943,10 @@ int rk4_step(nhqg_ctx* c) {
     e.s_out = dz[f];
     RUN(combine(c, nl[f], y[f], e));
   }
-  // th_bar: zero tendency, then its Dirichlet projection
-  LK(launch_small_axpy(Nq, c->ythb, c->ythb, c->ythb, 0.0, c->Rthb, c->st));
-  LK(launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->ythb, c->st));
+  // th_bar = proj_d (acc + dt/6 k4)
+  PL(CL_SMALL, 0.0, launch_small_lincomb(Nq, c->Rthb, 1.0, c->acctb, dt / 6.0, c->ktb, 0.0,
+                                         nullptr, 0.0, nullptr, c->st));
+  PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->ythb, c->st));
   RUN(sanitize(c));
   return NHQG_OK;
 }
@@ -1138,6 +1222,32 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   }
   RUN(upload(c, &o.sponge_phys, h->sponge_phys, (size_t)c->Nx * c->Nx));
   RUN(upload(c, (double**)&o.tw, h->twiddle, (size_t)2 * c->Nx));
+  // mean temperature: the legacy exchange needs all four maps, RK4 under
+  // evolve_mean the diffusion
+  if (p->evolve_mean && p->exchange == 1 &&
+      (!h->VGZ || !h->mean_solve || !h->mean_diff || !h->mean_flux)) {
+    c->err = "evolve_mean with the legacy exchange needs VGZ, mean_solve, mean_diff, mean_flux";
+    return NHQG_ERR_ARG;
+  }
+  RUN(upload(c, &o.vgz, h->VGZ, (size_t)Nq * Nq));
+  RUN(upload(c, &o.msolve, h->mean_solve, (size_t)Nq * Nq));
+  RUN(upload(c, &o.mdiff, h->mean_diff, (size_t)Nq * Nq));
+  RUN(upload(c, &o.mflux, h->mean_flux, (size_t)Nq * Nq));
+  // padded-grid horizontal means: the Nx-grid weights when not given
+  if (h->pweight_pad) {
+    RUN(upload_modes(c, &o.pw_pad, h->pweight_pad));
+  } else {
+    o.pw_pad = o.pw;
+  }
+  if (h->partner_pad) {
+    std::vector<int> part((size_t)c->nm_pad, -1);
+    std::memcpy(part.data(), h->partner_pad, sizeof(int) * nm);
+    RUN(upload(c, &o.partner_pad, part.data(), part.size()));
+  } else {
+    o.partner_pad = o.partner;
+  }
+  c->h_thb2s.assign(h->thb2s, h->thb2s + (size_t)Nq * Nq);
+  c->h_d1b.assign(h->d1_band, h->d1_band + (size_t)Nq * 3);
 
   // state and workspace
   RUN(field(c, &c->yq, Nq));
@@ -1184,7 +1294,9 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   c->Y6 = c->X5 + c->fstride * (p->advection ? 11 : 8);
   RUN(dalloc(c, &c->staging, (size_t)Nq * c->Nx * c->Nk));
   RUN(dalloc(c, &c->diag, (size_t)64 + 8 * Nq));
-  RUN(dalloc(c, &c->rowmax, (size_t)Nq * c->Nx));
+  RUN(dalloc(c, &c->rowmax, (size_t)4 * Nq * c->Nx));
+  double** vecs[] = {&c->mgrad, &c->etb1, &c->etb2, &c->itb, &c->ktb, &c->acctb, &c->stb};
+  for (double** v : vecs) RUN(dalloc(c, v, (size_t)Nq));
   CK(hipMemset(c->ythb, 0, sizeof(double) * Nq));
   CK(hipDeviceSynchronize());
   return NHQG_OK;
@@ -1248,7 +1360,12 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
   LK(launch_parseval(1, c->nm, c->ld, P, c->E2q, c->E2q, o.pw_ke, nullptr, c->partial, c->st));
   RUN(fetch(1));
   const double ke_bt = 0.5 * reduce_partials(part, 1, P, nullptr);
-  // max speed from the nodal psi
+  // w, theta at the CGL levels
+  RUN(gemm_jobs(c, {Job{&o.VSw, c->yw, c->wn, epi_make(1.0)},
+                    Job{&o.VSd, c->yth, c->thn, epi_make(1.0)}}));
+  // max speed from the nodal psi; max |w|, |theta|, |w theta| from the nodal
+  // w and theta, transformed in the same column pass
+  double maxes[4] = {0.0, 0.0, 0.0, 0.0};
   {
     FluxArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -1258,6 +1375,8 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
     a.nky = c->nky;
     a.ldc = c->ldc;
     a.qn = (const double2*)c->qn;
+    a.wn = (const double2*)c->wn;
+    a.thn = (const double2*)c->thn;
     a.kidx_of_x = o.kidx_of_x;
     a.rows = o.rows;
     a.kx_row = o.kx_row;
@@ -1268,14 +1387,13 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
     a.Y6 = c->Y6;
     a.fstride = c->fstride;
     LK(launch_speed(a, c->rowmax, c->st));
-    LK(launch_max_reduce((size_t)Nq * c->Nx, c->rowmax, c->diag, c->st));
+    const size_t sec = (size_t)Nq * c->Nx;
+    for (int k = 0; k < 4; ++k)
+      LK(launch_max_reduce(sec, c->rowmax + k * sec, c->diag + k, c->st));
   }
-  double max_speed = 0.0;
   CK(hipStreamSynchronize(c->st));
-  CK(hipMemcpy(&max_speed, c->diag, sizeof(double), hipMemcpyDeviceToHost));
-  // w, theta nodal
-  RUN(gemm_jobs(c, {Job{&o.VSw, c->yw, c->wn, epi_make(1.0)},
-                    Job{&o.VSd, c->yth, c->thn, epi_make(1.0)}}));
+  CK(hipMemcpy(maxes, c->diag, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  const double max_speed = maxes[0];
   LK(launch_parseval(Nq, c->nm, c->ld, P, c->wn, c->thn, o.pw, nullptr, c->partial, c->st));
   RUN(fetch(Nq));
   const double vol_tw = reduce_partials(part, Nq, P, cc.data());
@@ -1285,26 +1403,63 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
   LK(launch_parseval(Nq, c->nm, c->ld, P, c->thn, c->thn, o.pw, nullptr, c->partial, c->st));
   RUN(fetch(Nq));
   const double t2 = reduce_partials(part, Nq, P, cc.data());
-  // solver-consistent flux profile -> Nu_d (diagnostics.py:206-212, 655-675):
-  // vol = sum_j cc_j prof_j with prof = sbp_to_cgl <w_sbp th_sbp> on the SBP
-  // branch, so the host folds cc^T sbp_to_cgl into cc_sbp.
+  std::vector<double> tb(Nq);
+  CK(hipMemcpy(tb.data(), c->ythb, sizeof(double) * Nq, hipMemcpyDeviceToHost));
+  // solver-consistent flux profile -> Nu_d (diagnostics.py:206-212, 655-675),
+  // horizontal means on the padded 3/2 grid as there: on the SBP grid for the
+  // balanced_sbp2_pc exchange (vol = sum_j cc_j prof_j with prof = sbp_to_cgl
+  // <w_sbp th_sbp>, so the host folds cc^T sbp_to_cgl into cc_sbp), at the
+  // CGL levels otherwise
   double vol_d = 0.0;
+  double mon[5];
+  for (double& v : mon) v = std::numeric_limits<double>::quiet_NaN();
   {
     const double* wl = cc.data();
     std::vector<double> ccs(Nq);
-    if (c->P.evolve_mean) {
+    if (c->P.exchange == 0) {
       RUN(gemm_jobs(c, {Job{&o.W2S, c->yw, c->wsbp, epi_make(1.0)},
                         Job{&o.T2S, c->yth, c->thsbp, epi_make(1.0)}}));
-      LK(launch_parseval(Nq, c->nm, c->ld, P, c->wsbp, c->thsbp, o.pw, o.partner, c->partial,
-                         c->st));
+      LK(launch_parseval(Nq, c->nm, c->ld, P, c->wsbp, c->thsbp, o.pw_pad, o.partner_pad,
+                         c->partial, c->st));
       CK(hipMemcpy(ccs.data(), o.cc_sbp, sizeof(double) * Nq, hipMemcpyDeviceToHost));
       wl = ccs.data();
     } else {
-      LK(launch_parseval(Nq, c->nm, c->ld, P, c->wn, c->thn, o.pw, o.partner, c->partial,
+      LK(launch_parseval(Nq, c->nm, c->ld, P, c->wn, c->thn, o.pw_pad, o.partner_pad, c->partial,
                          c->st));
     }
     RUN(fetch(Nq));
     vol_d = reduce_partials(part, Nq, P, wl);
+    if (c->P.exchange == 0 && c->P.evolve_mean) {
+      // SBP-grid exchange monitor (diagnostics.py:334-369): the theta-side
+      // term -(D1 th_bar_sbp)^T H F and the mean-side term
+      // -th_bar_sbp^T H D1 F cancel up to the boundary term
+      std::vector<double> F(Nq), ts(Nq, 0.0);
+      for (int l = 0; l < Nq; ++l) {
+        double t = 0.0;
+        for (int q = 0; q < P; ++q) t += part[(size_t)l * P + q];
+        F[l] = t;
+      }
+      for (int i = 0; i < Nq; ++i)
+        for (int j = 0; j < Nq; ++j) ts[i] += c->h_thb2s[(size_t)i * Nq + j] * tb[j];
+      auto d1 = [&](const std::vector<double>& v, int i) {
+        double r = 0.0;
+        for (int k = -1; k <= 1; ++k)
+          if (i + k >= 0 && i + k < Nq) r += c->h_d1b[(size_t)i * 3 + k + 1] * v[i + k];
+        return r;
+      };
+      const double dz = 1.0 / (Nq - 1);
+      double fb = 0.0, ex = 0.0;
+      for (int i = 0; i < Nq; ++i) {
+        const double h = (i == 0 || i == Nq - 1) ? 0.5 * dz : dz;
+        fb -= d1(ts, i) * h * F[i];
+        ex -= ts[i] * h * d1(F, i);
+      }
+      mon[0] = fb;
+      mon[1] = ex;
+      mon[2] = ts[Nq - 1] * F[Nq - 1] - ts[0] * F[0];
+      mon[3] = fb + ex;
+      mon[4] = mon[3] / std::max(std::fabs(fb) + std::fabs(ex), 1e-300);
+    }
   }
   // vertical spectra: raw sums of |f_n|^2 of q, w_cheb, th_cheb
   // (diagnostics.py:583-585)
@@ -1322,10 +1477,16 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
     }
   }
   double thb_max = 0.0;
-  {
-    std::vector<double> tb(Nq);
-    CK(hipMemcpy(tb.data(), c->ythb, sizeof(double) * Nq, hipMemcpyDeviceToHost));
-    for (double v : tb) thb_max = std::max(thb_max, std::fabs(v));
+  for (double v : tb) thb_max = std::max(thb_max, std::fabs(v));
+  // energy fraction of the top quarter of each vertical spectrum
+  // (high_mode_fraction, diagnostics.py:590-596)
+  double hf[3];
+  for (int f = 0; f < 3; ++f) {
+    const int n_tail = std::max(1, Nq / 4);
+    double tot = 0.0, tail = 0.0;
+    for (int l = 0; l < Nq; ++l) tot += spec[(size_t)f * Nq + l];
+    for (int l = Nq - n_tail; l < Nq; ++l) tail += spec[(size_t)f * Nq + l];
+    hf[f] = tot > 0.0 ? tail / tot : 0.0;
   }
   for (int i = 0; i < n_out; ++i) out[i] = 0.0;
   out[NHQG_DIAG_KE_TOT] = ke_tot;
@@ -1341,8 +1502,20 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
   out[NHQG_DIAG_Q_RMS] = std::sqrt(q2);
   out[NHQG_DIAG_W_RMS] = std::sqrt(w2);
   out[NHQG_DIAG_TH_RMS] = std::sqrt(t2);
-  for (int i = 0; i < NHQG_DIAG_SCALARS + 3 * Nq; ++i)
+  out[NHQG_DIAG_MAX_W] = maxes[1];
+  out[NHQG_DIAG_MAX_THETA] = maxes[2];
+  out[NHQG_DIAG_MAX_TW] = maxes[3];
+  out[NHQG_DIAG_VOLTW] = vol_tw;
+  out[NHQG_DIAG_HEAT_FLUX_MISMATCH] = vol_d - vol_tw;
+  out[NHQG_DIAG_Q_HIGH_FRAC] = hf[0];
+  out[NHQG_DIAG_W_HIGH_FRAC] = hf[1];
+  out[NHQG_DIAG_TH_HIGH_FRAC] = hf[2];
+  for (int k = 0; k < 5; ++k) out[NHQG_DIAG_MON_FEEDBACK + k] = mon[k];
+  const bool mon_on = c->P.exchange == 0 && c->P.evolve_mean;
+  for (int i = 0; i < NHQG_DIAG_SCALARS + 3 * Nq; ++i) {
+    if (!mon_on && i >= NHQG_DIAG_MON_FEEDBACK && i <= NHQG_DIAG_MON_RESIDUAL_REL) continue;
     if (!std::isfinite(out[i])) return NHQG_NONFINITE;
+  }
   return NHQG_OK;
 }
 
@@ -1452,6 +1625,29 @@ int nhqg_diagnostics(nhqg_ctx* c, double* out, int32_t n_out) {
   if (!c || !out) return NHQG_ERR_ARG;
   if (!c->have_state) return NHQG_ERR_STATE;
   return diagnostics_impl(c, out, n_out);
+}
+
+int nhqg_mode_power(nhqg_ctx* c, double* out, int32_t n_out) {
+  if (!c || !out) return NHQG_ERR_ARG;
+  if (!c->have_state) return NHQG_ERR_STATE;
+  if (n_out < 3 * c->nm) {
+    c->err = "mode power output shorter than 3 * nm";
+    return NHQG_ERR_ARG;
+  }
+  const Ops& o = c->o;
+  if (!c->mpow) RUN(dalloc(c, &c->mpow, (size_t)3 * c->nm_pad));
+  RUN(gemm_jobs(c, {Job{&o.V, c->yq, c->qn, epi_make(1.0)},
+                    Job{&o.VSw, c->yw, c->wn, epi_make(1.0)},
+                    Job{&o.VSd, c->yth, c->thn, epi_make(1.0)}}));
+  const double* f[3] = {c->qn, c->wn, c->thn};
+  for (int k = 0; k < 3; ++k)
+    LK(launch_depth_power(c->Nq, c->ld, c->nm, o.cc, f[k], c->mpow + (size_t)k * c->nm_pad,
+                          c->st));
+  CK(hipStreamSynchronize(c->st));
+  for (int k = 0; k < 3; ++k)
+    CK(hipMemcpy(out + (size_t)k * c->nm, c->mpow + (size_t)k * c->nm_pad,
+                 sizeof(double) * c->nm, hipMemcpyDeviceToHost));
+  return NHQG_OK;
 }
 
 static int apply_impl(nhqg_ctx* c, int32_t op, double arg);

@@ -988,3 +988,94 @@ hipError_t launch_stream_copy(size_t n16, const double2* src, double2* dst, hipS
   hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, s, n16, src, dst);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// 1-D and per-level helpers of the mean-temperature equation and the
+// diagnostics (evolve_mean with the legacy exchange, the RK4 validator).
+// ---------------------------------------------------------------------------
+
+// y = c0 x0 + c1 x1 + c2 x2 + c3 x3 over n entries (null x_k skipped)
+__global__ void __launch_bounds__(256)
+small_lincomb_kernel(int n, double* __restrict__ y, double c0, const double* x0, double c1,
+                     const double* x1, double c2, const double* x2, double c3, const double* x3) {
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double v = x0 ? c0 * x0[i] : 0.0;
+    if (x1) v += c1 * x1[i];
+    if (x2) v += c2 * x2[i];
+    if (x3) v += c3 * x3[i];
+    y[i] = v;
+  }
+}
+
+hipError_t launch_small_lincomb(int n, double* y, double c0, const double* x0, double c1,
+                                const double* x1, double c2, const double* x2, double c3,
+                                const double* x3, hipStream_t s) {
+  hipLaunchKernelGGL(small_lincomb_kernel, dim3(1), dim3(256), 0, s, n, y, c0, x0, c1, x1, c2, x2,
+                     c3, x3);
+  return hipGetLastError();
+}
+
+// y = M v with v[l] = sum_p partial[l][p] (the per-level horizontal means of
+// launch_parseval), one block; both sums in a fixed order
+__global__ void __launch_bounds__(256)
+partial_matvec_kernel(int n_out, int nl, int P, const double* __restrict__ partial,
+                      const double* __restrict__ M, double* __restrict__ y) {
+  __shared__ double v[SBP_NMAX];
+  for (int l = threadIdx.x; l < nl; l += 256) {
+    double t = 0.0;
+    for (int p = 0; p < P; ++p) t += partial[(size_t)l * P + p];
+    v[l] = t;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_out; i += 256) {
+    double t = 0.0;
+    for (int l = 0; l < nl; ++l) t += M[(size_t)i * nl + l] * v[l];
+    y[i] = t;
+  }
+}
+
+hipError_t launch_partial_matvec(int n_out, int nl, int P, const double* partial, const double* M,
+                                 double* y, hipStream_t s) {
+  if (nl > SBP_NMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(partial_matvec_kernel, dim3(1), dim3(256), 0, s, n_out, nl, P, partial, M, y);
+  return hipGetLastError();
+}
+
+// y[l][i] = g[l] x[l][i] over the ld doubles of each level
+__global__ void __launch_bounds__(256)
+level_scale_kernel(int ld, const double* __restrict__ g, const double* __restrict__ x,
+                   double* __restrict__ y) {
+  const int l = blockIdx.y;
+  const double a = g[l];
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < ld; i += gridDim.x * 256)
+    y[(size_t)l * ld + i] = a * x[(size_t)l * ld + i];
+}
+
+hipError_t launch_level_scale(int nl, int ld, const double* g, const double* x, double* y,
+                              hipStream_t s) {
+  const int nb = (ld + 255) / 256 < 64 ? (ld + 255) / 256 : 64;
+  hipLaunchKernelGGL(level_scale_kernel, dim3(nb, nl), dim3(256), 0, s, ld, g, x, y);
+  return hipGetLastError();
+}
+
+// out[m] = sum_l cc[l] |f[l][m]|^2, the depth-integrated power of each mode
+// of a nodal field (diagnostics.py:99-108 before the shell binning)
+__global__ void __launch_bounds__(256)
+depth_power_kernel(int nl, int ld, int nm, const double* __restrict__ cc,
+                   const double* __restrict__ f, double* __restrict__ out) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= nm) return;
+  double s = 0.0;
+  for (int l = 0; l < nl; ++l) {
+    const double re = f[(size_t)l * ld + 2 * m], im = f[(size_t)l * ld + 2 * m + 1];
+    s += cc[l] * (re * re + im * im);
+  }
+  out[m] = s;
+}
+
+hipError_t launch_depth_power(int nl, int ld, int nm, const double* cc, const double* f,
+                              double* out, hipStream_t s) {
+  hipLaunchKernelGGL(depth_power_kernel, dim3((nm + 255) / 256), dim3(256), 0, s, nl, ld, nm, cc, f,
+                     out);
+  return hipGetLastError();
+}

@@ -150,6 +150,8 @@ struct FluxArgs {
 };
 // pass: 0 all three passes, 1 col_inverse, 2 row products, 3 col_forward
 hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s);
+// rowmax: Nq*N row maxima of the speed from psi (a.qn); with a.wn and a.thn
+// set, three more Nq*N sections: max |w|, max |theta|, max |w theta|
 hipError_t launch_speed(const FluxArgs& a, double* rowmax, hipStream_t s);
 // Shell matrices for launch_shell_solve: order np = 64*ceil(Ng/64), shell s
 // at s*np*np, inv_s[j][i] at j*np + i (the transposed inverse), zero padded.
@@ -192,6 +194,15 @@ hipError_t launch_gather(int nl, int nm, int nm_pad, int full_n, const int* full
 hipError_t launch_scatter(int nl, int nm, int nm_pad, int full_n, const int* full_idx,
                           const double2* comp, double2* full, hipStream_t s);
 hipError_t launch_max_reduce(size_t n, const double* x, double* out, hipStream_t s);
+hipError_t launch_small_lincomb(int n, double* y, double c0, const double* x0, double c1,
+                                const double* x1, double c2, const double* x2, double c3,
+                                const double* x3, hipStream_t s);
+hipError_t launch_partial_matvec(int n_out, int nl, int P, const double* partial, const double* M,
+                                 double* y, hipStream_t s);
+hipError_t launch_level_scale(int nl, int ld, const double* g, const double* x, double* y,
+                              hipStream_t s);
+hipError_t launch_depth_power(int nl, int ld, int nm, const double* cc, const double* f,
+                              double* out, hipStream_t s);
 hipError_t launch_stream_copy(size_t n16, const double2* src, double2* dst, hipStream_t s);
 hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mode,
                        const double* inv_denom, const double2* q, double2* eq, hipStream_t s);

@@ -836,6 +836,53 @@ row_speed(int nky, const double2* __restrict__ X5, size_t fstride, double* __res
   }
 }
 
+// max |w|, |theta| and |w theta| over each grid row x of one level, from the
+// column transforms of the nodal w and theta that the speed pass leaves in
+// X5 fields 3 and 4 (compute_diagnostics' max_w, max_theta, max_tw,
+// nhqg/diagnostics.py:640-643); rowmax[k * rstride + l * N + x], k = w, theta,
+// w theta.  A NaN anywhere in a row makes that row's maxima NaN.
+template <int N>
+__global__ void __launch_bounds__(256)
+row_wth(int nky, const double2* __restrict__ X5, size_t fstride, double* __restrict__ rowmax,
+        size_t rstride, const double2* __restrict__ twg) {
+  constexpr int T = N / 4, TEAMS = 256 / T;
+  __shared__ double2 buf[2][TEAMS][N + FFT_PAD];
+  __shared__ double2 tw[N];
+  __shared__ double red[3][256];
+  const int tid = threadIdx.x, team = tid / T, t = tid % T;
+  const int l = blockIdx.y;
+  int x = blockIdx.x * TEAMS + team;
+  const bool active = x < N;
+  if (!active) x = N - 1;
+  load_twiddles<N>(tw, twg);
+  __syncthreads();
+  const size_t ro = ((size_t)l * N + x) * nky;
+  const double sc = 1.0 / ((double)N * (double)N);
+  double w[4], th[4];
+  inv_pair<N>(buf[0][team], buf[1][team], t, tw, nky, X5 + 3 * fstride + ro,
+              X5 + 4 * fstride + ro, w, th, sc);
+  auto nmax = [](double a, double b) { return (a != a || b != b) ? a + b : (a > b ? a : b); };
+  double mw = 0.0, mt = 0.0, mwt = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    mw = nmax(mw, fabs(w[j]));
+    mt = nmax(mt, fabs(th[j]));
+    mwt = nmax(mwt, fabs(w[j] * th[j]));
+  }
+  red[0][tid] = mw;
+  red[1][tid] = mt;
+  red[2][tid] = mwt;
+  __syncthreads();
+  if (t == 0 && active) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double r = red[k][tid];
+      for (int i = 1; i < T; ++i) r = nmax(r, red[k][tid + i]);
+      rowmax[k * rstride + (size_t)l * N + x] = r;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -907,11 +954,17 @@ static hipError_t speed_n(const FluxArgs& a, double* rowmax, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;
   constexpr int BC = ColBlock<N>::v, TC = 4 * BC / N;
   const int nb = (a.nky + TC - 1) / TC * a.nq;
+  // X5 fields 0, 1: u, v from psi; 3, 4: the nodal w and theta when given
+  const double2* wn = a.wn ? a.wn : a.qn;
+  const double2* thn = a.thn ? a.thn : a.qn;
   hipLaunchKernelGGL((col_inverse_flux<N, false, false>), xcd_grid(nb), dim3(BC), 0, s, a.nkx,
-                     a.nky, a.ldc, 0, a.qn, a.qn, a.qn, a.qn, a.qn, a.qn, a.kidx_of_x, a.kx_row,
+                     a.nky, a.ldc, 0, a.qn, wn, thn, a.qn, a.qn, a.qn, a.kidx_of_x, a.kx_row,
                      a.ky_col, a.inv_denom, a.tw, a.X5, a.fstride, nb);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
   hipLaunchKernelGGL(row_speed<N>, gr, dim3(256), 0, s, a.nky, a.X5, a.fstride, rowmax, a.tw);
+  if (a.wn && a.thn)
+    hipLaunchKernelGGL(row_wth<N>, gr, dim3(256), 0, s, a.nky, a.X5, a.fstride,
+                       rowmax + (size_t)a.nq * N, (size_t)a.nq * N, a.tw);
   return hipGetLastError();
 }
 

@@ -388,9 +388,10 @@ def _check_config(cfg: NHQGConfig) -> None:
         if getattr(cfg, name) != want:
             raise NotImplementedError(
                 f"the MI355X step is built for {name}={want!r} (got {getattr(cfg, name)!r})")
-    if cfg.thermal_closure == "evolve_mean" and cfg.mean_exchange_discretization != "balanced_sbp2_pc":
+    if cfg.mean_exchange_discretization not in ("legacy", "balanced_sbp2_pc"):
         raise NotImplementedError(
-            "evolve_mean is built with mean_exchange_discretization='balanced_sbp2_pc'")
+            "the MI355X step is built for mean_exchange_discretization='balanced_sbp2_pc' or "
+            f"'legacy' (got {cfg.mean_exchange_discretization!r})")
     if cfg.vertical_cutoff_n is not None and cfg.vertical_cutoff_n < cfg.Nz:
         raise NotImplementedError("vertical_cutoff_n is not built")
     if cfg.Nx < 16 or cfg.Nx > 1024 or cfg.Nx & (cfg.Nx - 1):
@@ -571,6 +572,13 @@ class ModeLayout:
         pairs = np.stack([m_all[pm], partner[pm]], 1)
         pairs = pairs[pairs[:, 0] <= pairs[:, 1]].astype(np.int32)
         wgt = np.where(self_conj, 1.0, 2.0) / float(Nx) ** 4
+        # the same means on the zero-padded 3/2 grid (the reference's
+        # horizontal means with Npad): there ky = Nx/2 is an ordinary column
+        # (weight 2, no partner) and the kx = Nx/2 row of ky = 0 sits at
+        # -Nx/2 with no partner (Hermitian part of the column: weight 1/2)
+        nyq_row = (Nx % 2 == 0) & (row_m == Nx // 2)
+        wgt_pad = np.where(ky_i == 0, np.where(nyq_row, 0.5, 1.0), 2.0) / float(Nx) ** 4
+        partner_pad = np.where((ky_i == 0) & ~nyq_row, partner, -1).astype(np.int32)
         # shells touched by the layout
         shell_of_m = g.ksq_idx[sel].astype(np.int64)
         shell_ids, inv = np.unique(shell_of_m, return_inverse=True)
@@ -584,7 +592,8 @@ class ModeLayout:
             inv_denom=g.inv_denom[sel].copy(), inv_alpha_q=g.inv_alpha_q[sel].copy(),
             inv_alpha_th=g.inv_alpha_th[sel].copy(), diss_q=g.diss_rate_q[sel].copy(),
             diss_w=g.diss_rate_w[sel].copy(), diss_th=g.diss_rate_th[sel].copy(),
-            mask=g.mask_23[sel].copy(), pweight=wgt, partner=partner,
+            mask=g.mask_23[sel].copy(), pweight=wgt, partner=partner, pweight_pad=wgt_pad,
+            partner_pad=partner_pad,
             herm_pairs=np.ascontiguousarray(pairs.ravel()),
             shell_ids=shell_ids, shell_ptr=shell_ptr,
             shell_modes=order.astype(np.int32),
@@ -669,6 +678,17 @@ def device_operators(g: Grid, lay: ModeLayout) -> dict:
         ops["eta_x_phys"] = np.fft.irfft2(1j * g.kx * g.eta_hat, s=(Nx, Nx))
         ops["eta_y_phys"] = np.fft.irfft2(1j * g.ky * g.eta_hat, s=(Nx, Nx))
     ops["sponge_phys"] = g.sponge_phys
+    # mean temperature (nhqg/solver.py:575-590, 638, 746-758)
+    N = g.Nz
+    a_m = g.gamma_imex * g.dt * g.mean_temp_eps_sq / g.sigma
+    A = np.eye(N + 1) - a_m * g.G_Z2
+    A[N - 1, :] = 1.0
+    A[N, :] = (-1.0) ** np.arange(N + 1)
+    ms = np.linalg.inv(A)
+    ms[:, N - 1:] = 0.0          # the tau rows of the right-hand side are dropped
+    ops.update(VGZ=V @ GZ, mean_solve=ms,
+               mean_diff=(g.mean_temp_eps_sq / g.sigma) * g.G_Z2,
+               mean_flux=-g.mean_temp_eps_sq * (GZ @ Vi))
     k = np.arange(Nx)
     ops["twiddle"] = np.stack([np.cos(2.0 * np.pi * k / Nx), -np.sin(2.0 * np.pi * k / Nx)], 1)
     # shell factors of the touched shells: built and inverted on the device

@@ -68,8 +68,7 @@ def unsupported_reason(ref) -> str | None:
     for name, v in want.items():
         if getattr(ref, name) != v:
             return f"{name}={getattr(ref, name)!r}"
-    if ref.thermal_closure == "evolve_mean" and \
-            ref.mean_exchange_discretization != "balanced_sbp2_pc":
+    if ref.mean_exchange_discretization not in ("legacy", "balanced_sbp2_pc"):
         return f"mean_exchange_discretization={ref.mean_exchange_discretization!r}"
     if ref.vertical_cutoff_n is not None and ref.vertical_cutoff_n < ref.Nz:
         return "vertical_cutoff_n"

@@ -185,9 +185,9 @@ class Stepper:
         return State(*self._current().get_state())
 
     def diagnostics(self) -> dict:
-        from nhqg_amd.diagnostics import unpack
+        from nhqg_amd.diagnostics import device_diagnostics
 
-        return unpack(self._current().diagnostics(), self.grid.Nz + 1)
+        return device_diagnostics(self._current(), self.grid)
 
     def close(self) -> None:
         for c in (self.ctx, self._full):

@@ -1,12 +1,12 @@
 """RK4 validator on the GPU (nhqg_amd.rk4_step, solver.py:1276-1336)
 against the oracle's restatement, and the reference's IMEX-vs-RK4 test run
-on the device.  fixed_conduction configs; evolve_mean is rejected."""
+on the device: fixed_conduction configs, and evolve_mean with the split
+(balanced_sbp2_pc: mean diffusion only) and the legacy exchange."""
 
 import numpy as np
 import pytest
 
 from nhqg_amd import NHQGConfig, State, imex_step, make_grid, rk4_step
-from nhqg_amd._native import NativeError
 from oracle import nhqg_oracle as O
 from tests import cases
 
@@ -30,6 +30,11 @@ CONFIGS = {
     "fixed_neumann": lambda: cases.neumann_cfg(**FIXED),
     "fixed_trap_sponge": lambda: cases.trap_sponge_cfg(**FIXED),
     "fixed_jacobian": lambda: cases.fixture_cfg(nonlinear_advection="jacobian", **FIXED),
+    # explicit mean diffusion: dt inside RK4's stability region at Nz = 12-16
+    "evolve_pc": lambda: cases.fixture_cfg(dt=2e-5),
+    "evolve_legacy": lambda: cases.fixture_cfg(mean_exchange_discretization="legacy", dt=2e-5),
+    "evolve_legacy_trap_sponge": lambda: cases.trap_sponge_cfg(
+        mean_exchange_discretization="legacy", dt=2e-5),
 }
 
 
@@ -66,13 +71,4 @@ def test_imex_and_rk4_agree_on_device():
     r = rel(a.q_hat, b.q_hat)
     say("imex vs rk4", r)
     assert 0.0 < r < 1e-3
-    g.close()
-
-
-def test_rk4_rejects_evolve_mean():
-    cfg = cases.fixture_cfg()
-    g = make_grid(cfg)
-    s = cases.active_state(cfg)
-    with pytest.raises(NativeError):
-        rk4_step(State(s.q, s.w, s.th, s.thb), g)
     g.close()
