@@ -217,6 +217,14 @@ int nhqg_diagnostics(nhqg_ctx *ctx, double *out, int32_t n_out);
  * (nhqg/diagnostics.py:99-108, 683-685). */
 int nhqg_mode_power(nhqg_ctx *ctx, double *out, int32_t n_out);
 
+/* dst's state <- src's state spectrally upsampled from src's nx to dst's
+ * (same nz, dst nx >= src nx, same parity), then sanitized for dst's layout:
+ * the method of scripts/upsample_checkpoint_horiz.py:27-59 (the old Nyquist
+ * row and column split in half, factor (nx_dst/nx_src)^2), computed on the
+ * device from the resident state with no host round trip.  Both contexts
+ * on the current device; th_bar is copied. */
+int nhqg_upsample(nhqg_ctx *dst, nhqg_ctx *src);
+
 /* Component entry points on the device state, for parity testing:
  *   NHQG_OP_EXPLICIT_RHS    aux <- explicit_rhs(state)    solver.py:510-599
  *   NHQG_OP_IMPLICIT_TEND   aux <- implicit_tendency(state) solver.py:615-641

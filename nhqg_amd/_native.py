@@ -75,7 +75,7 @@ EXPORTS = (
     "nhqg_mlp_create", "nhqg_mlp_destroy", "nhqg_mlp_n_params", "nhqg_mlp_set_params",
     "nhqg_mlp_get_params", "nhqg_mlp_forward", "nhqg_mlp_gradient", "nhqg_mlp_set_data",
     "nhqg_mlp_train", "nhqg_mlp_last_error", "nhqg_set_closure", "nhqg_mlp_forward_noalloc",
-    "nhqg_mode_power",
+    "nhqg_mode_power", "nhqg_upsample",
 )
 
 
@@ -106,6 +106,7 @@ def load():
     lib.nhqg_sync.argtypes = [vp]
     lib.nhqg_diagnostics.argtypes = [vp, _dp, C.c_int32]
     lib.nhqg_mode_power.argtypes = [vp, _dp, C.c_int32]
+    lib.nhqg_upsample.argtypes = [vp, vp]
     lib.nhqg_apply.argtypes = [vp, C.c_int32, C.c_double]
     lib.nhqg_get_aux.argtypes = [vp, _dp, _dp, _dp]
     lib.nhqg_last_step_ms.argtypes = [vp, _dp]
@@ -272,6 +273,10 @@ class Context:
         if rc != NHQG_NONFINITE:
             self._check(rc, "nhqg_diagnostics")
         return out
+
+    def upsample_from(self, src: "Context") -> None:
+        """This context's state <- src's state spectrally upsampled (device)."""
+        self._check(self.lib.nhqg_upsample(self.h, src.h), "nhqg_upsample")
 
     def mode_power(self):
         """(3, nm): sum_j cc_j |f_j|^2 of the nodal q, w, theta per mode."""

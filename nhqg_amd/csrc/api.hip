@@ -1627,6 +1627,32 @@ int nhqg_diagnostics(nhqg_ctx* c, double* out, int32_t n_out) {
   return diagnostics_impl(c, out, n_out);
 }
 
+int nhqg_upsample(nhqg_ctx* c, nhqg_ctx* src) {
+  if (!c || !src) return NHQG_ERR_ARG;
+  if (!src->have_state) {
+    c->err = "nhqg_upsample: the source context has no state";
+    return NHQG_ERR_STATE;
+  }
+  if (src->Nq != c->Nq || c->Nx < src->Nx || (c->Nx - src->Nx) % 2) {
+    c->err = "nhqg_upsample: needs the same nz and a destination nx >= source nx of the same "
+             "parity";
+    return NHQG_ERR_ARG;
+  }
+  CK(hipStreamSynchronize(src->st));
+  const double scale = ((double)c->Nx / src->Nx) * ((double)c->Nx / src->Nx);
+  const double* sf[3] = {src->yq, src->yw, src->yth};
+  double* df[3] = {c->yq, c->yw, c->yth};
+  const int nl[3] = {c->Nq, c->Ng, c->Ng};
+  for (int f = 0; f < 3; ++f)
+    LK(launch_upsample(nl[f], c->nm, c->nkx, c->Nx, c->o.rows, c->ld, df[f], src->Nx, src->nkx,
+                       src->nky, src->o.kidx_of_x, src->o.partner, src->ld, sf[f], scale, c->st));
+  CK(hipMemcpyAsync(c->ythb, src->ythb, sizeof(double) * c->Nq, hipMemcpyDeviceToDevice, c->st));
+  RUN(sanitize(c));
+  CK(hipStreamSynchronize(c->st));
+  c->have_state = true;
+  return NHQG_OK;
+}
+
 int nhqg_mode_power(nhqg_ctx* c, double* out, int32_t n_out) {
   if (!c || !out) return NHQG_ERR_ARG;
   if (!c->have_state) return NHQG_ERR_STATE;

@@ -1079,3 +1079,58 @@ hipError_t launch_depth_power(int nl, int ld, int nm, const double* cc, const do
                      out);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Horizontal spectral upsampling Nx -> Nd (Nd >= Nx, same parity), the
+// method of scripts/upsample_checkpoint_horiz.py:27-59 written as a direct
+// mode map: the real field's spectrum (the self-conjugate columns of the
+// source projected onto their Hermitian part, as irfft2 does) scaled by
+// (Nd/Nx)^2, its Nyquist row kx = -Nx/2 split in half between +-Nx/2 and its
+// Nyquist column ky = Nx/2 halved, every other destination mode zero.
+// One thread per (level, destination mode) of the destination layout.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+upsample_kernel(int nm_d, int nkx_d, int nd, const int* __restrict__ rows_d, int ld_d,
+                double* __restrict__ out, int nx, int nkx_s, int nky_s,
+                const int* __restrict__ kidx_s, const int* __restrict__ partner_s, int ld_s,
+                const double* __restrict__ src, double scale) {
+  const int l = blockIdx.y;
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= nm_d) return;
+  const int ky = m / nkx_d;
+  const int row = rows_d[m % nkx_d];
+  const int kx = row < nd / 2 ? row : row - nd;   // fftfreq order
+  double re = 0.0, im = 0.0;
+  const int akx = kx < 0 ? -kx : kx;
+  if (akx <= nx / 2 && ky <= nx / 2 && ky < nky_s) {
+    const int srow = ((kx % nx) + nx) % nx;
+    const int si = kidx_s[srow];
+    if (si >= 0) {
+      const int ms = ky * nkx_s + si;
+      const double* f = src + (size_t)l * ld_s;
+      re = f[2 * ms];
+      im = f[2 * ms + 1];
+      const int p = partner_s[ms];
+      if (p >= 0) {   // self-conjugate column: its Hermitian part
+        re = 0.5 * (re + f[2 * p]);
+        im = 0.5 * (im - f[2 * p + 1]);
+      }
+      double fac = scale;
+      if ((nx % 2 == 0) && akx == nx / 2) fac *= 0.5;
+      if ((nx % 2 == 0) && ky == nx / 2) fac *= 0.5;
+      re *= fac;
+      im *= fac;
+    }
+  }
+  out[(size_t)l * ld_d + 2 * m] = re;
+  out[(size_t)l * ld_d + 2 * m + 1] = im;
+}
+
+hipError_t launch_upsample(int nl, int nm_d, int nkx_d, int nd, const int* rows_d, int ld_d,
+                           double* out, int nx, int nkx_s, int nky_s, const int* kidx_s,
+                           const int* partner_s, int ld_s, const double* src, double scale,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(upsample_kernel, dim3((nm_d + 255) / 256, nl), dim3(256), 0, s, nm_d, nkx_d,
+                     nd, rows_d, ld_d, out, nx, nkx_s, nky_s, kidx_s, partner_s, ld_s, src, scale);
+  return hipGetLastError();
+}

@@ -203,6 +203,10 @@ hipError_t launch_level_scale(int nl, int ld, const double* g, const double* x, 
                               hipStream_t s);
 hipError_t launch_depth_power(int nl, int ld, int nm, const double* cc, const double* f,
                               double* out, hipStream_t s);
+hipError_t launch_upsample(int nl, int nm_d, int nkx_d, int nd, const int* rows_d, int ld_d,
+                           double* out, int nx, int nkx_s, int nky_s, const int* kidx_s,
+                           const int* partner_s, int ld_s, const double* src, double scale,
+                           hipStream_t s);
 hipError_t launch_stream_copy(size_t n16, const double2* src, double2* dst, hipStream_t s);
 hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mode,
                        const double* inv_denom, const double2* q, double2* eq, hipStream_t s);

@@ -189,6 +189,13 @@ class Stepper:
 
         return device_diagnostics(self._current(), self.grid)
 
+    def upsample_from(self, other: "Stepper") -> None:
+        """This Stepper's state <- other's resident state spectrally upsampled
+        to this grid on the device (same Nz, Nx at least other's; the method
+        of scripts/upsample_checkpoint_horiz.py:27-59), then sanitized."""
+        self._band().upsample_from(other._current())
+        self._pending_full = False
+
     def close(self) -> None:
         for c in (self.ctx, self._full):
             if c is not None:
