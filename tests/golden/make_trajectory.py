@@ -17,6 +17,13 @@ Cases:
            (nhqg_amd.init.polar_state_256, seed 0); hours of CPU time
 
     python tests/golden/make_trajectory.py --case c1 --workers 4
+
+--probes1 writes tests/golden/<case>_step1_probes.npz instead: after one
+oracle step, 4096 seeded random in-band coefficients of each of q, w,
+theta, each field's largest magnitude, and the whole th_bar (full-field
+per-step parity at the case's size without storing the fields).
+
+    python tests/golden/make_trajectory.py --case c2 --probes1 --workers 8
 """
 
 from __future__ import annotations
@@ -73,16 +80,41 @@ def summarize(q, w, th, thb, diag: dict, step: int) -> dict:
     return rec
 
 
+N_PROBES = 4096
+
+
+def step1_probes(case: str, s, o) -> int:
+    t0 = time.time()
+    s1 = O.imex_step(s, o)
+    rng = np.random.default_rng(2024)
+    kept = np.argwhere(o.mask > 0)
+    out = {"th_bar": s1.thb}
+    for name, f in (("q_hat", s1.q), ("w_hat", s1.w), ("th_hat", s1.th)):
+        lev = rng.integers(0, f.shape[0], N_PROBES)
+        pick = kept[rng.integers(0, kept.shape[0], N_PROBES)]
+        idx = np.stack([lev, pick[:, 0], pick[:, 1]], 1).astype(np.int16)
+        out[name + "_idx"] = idx
+        out[name + "_val"] = f[idx[:, 0], idx[:, 1], idx[:, 2]]
+        out[name + "_max"] = np.abs(f).max()
+    path = os.path.join(HERE, f"{case}_step1_probes.npz")
+    np.savez_compressed(path, **out)
+    print(f"{case}: one oracle step and probes in {time.time() - t0:.0f} s -> {path}")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--case", required=True, choices=["c1", "c1_flow", "c2"])
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--probes1", action="store_true")
     args = ap.parse_args()
     O.set_workers(args.workers)
     cfg, st = case_state(args.case)
     o = O.build_operators(cfg)
     s = O.OState(st.q_hat.copy(), st.w_hat.copy(), st.th_hat.copy(), st.th_bar.copy())
+    if args.probes1:
+        return step1_probes(args.case, s, o)
     out_path = os.path.join(HERE, f"{args.case}_trajectory.json")
     recs = [summarize(s.q, s.w, s.th, s.thb, O.diagnostics(s, o), 0)]
     t0 = time.time()

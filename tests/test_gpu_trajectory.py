@@ -79,3 +79,34 @@ def test_trajectory_matches_oracle(case):
         tb = float(np.sqrt(np.sum(s.th_bar ** 2)))
         assert abs(tb - rec["th_bar_l2"]) <= TOL * max(rec["th_bar_l2"], 1e-300) + 1e-30
     say(case, "worst relative errors", {k: f"{v:.2e}" for k, v in worst.items()})
+
+
+def test_c2_one_step_full_field_probes():
+    """Full-field per-step parity at 256^2 x 256: after one device step from
+    the c2 state, 4096 seeded random in-band coefficients of each of q, w,
+    theta within 1e-12 of that field's largest magnitude, and the whole
+    th_bar within 1e-12 (tests/golden/c2_step1_probes.npz, written by
+    make_trajectory.py --probes1 from one oracle step)."""
+    from nhqg_amd import imex_step
+    from tests.golden.make_trajectory import case_state
+
+    path = os.path.join(HERE, "golden", "c2_step1_probes.npz")
+    if not os.path.exists(path):
+        pytest.skip("no c2 step-1 probe fixture")
+    fx = np.load(path)
+    cfg, st = case_state("c2")
+    g = make_grid(cfg, shells="band")
+    say("c2 one step, full-field probes")
+    s1 = imex_step(State(st.q_hat, st.w_hat, st.th_hat, st.th_bar), g)
+    worst = {}
+    for name, f in (("q_hat", s1.q_hat), ("w_hat", s1.w_hat), ("th_hat", s1.th_hat)):
+        idx = fx[name + "_idx"].astype(np.int64)
+        got = f[idx[:, 0], idx[:, 1], idx[:, 2]]
+        scale = float(fx[name + "_max"])
+        e = float(np.abs(got - fx[name + "_val"]).max() / scale)
+        worst[name] = e
+        assert e <= 1e-12, (name, e)
+    tb = fx["th_bar"]
+    worst["th_bar"] = float(np.abs(s1.th_bar - tb).max() / np.abs(tb).max())
+    assert worst["th_bar"] <= 1e-12
+    say("c2 step-1 worst relative errors", {k: f"{v:.2e}" for k, v in worst.items()})
