@@ -162,7 +162,36 @@ def cpu_baseline(cfg, st, steps: int, gpu_one_step):
             "sample": f"{steps} step(s) of the 256^2x256 workload from the same initial state; "
                       f"oracle operator build excluded; scipy.fft workers={cores}, "
                       f"BLAS default threads; CPU: {model}"}
+    base["c1"] = cpu_c1(cores)
     return base, par
+
+
+def cpu_c1(cores: int, steps: int = 200) -> dict:
+    """The CPU-runnable 64^2 x 32 workload, 200 oracle steps on all host
+    cores and on one thread (BLAS and FFT limited to one)."""
+    from threadpoolctl import threadpool_limits
+
+    from nhqg_amd import make_grid
+    from nhqg_amd.init import cpu_config_64, thermal_noise_state
+    from oracle import nhqg_oracle as O
+
+    cfg = cpu_config_64()
+    st = thermal_noise_state(make_grid(cfg, shells="band"), seed=0, amplitude=1e-3)
+    o = O.build_operators(cfg)
+    out = {}
+    for label, n in (("all_cores", cores), ("one_thread", 1)):
+        O.set_workers(n)
+        s = O.OState(st.q_hat.copy(), st.w_hat.copy(), st.th_hat.copy(), st.th_bar.copy())
+        with threadpool_limits(n):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s = O.imex_step(s, o)
+            dt = time.perf_counter() - t0
+        out[label] = {"value": steps / dt, "unit": "steps/s", "threads": n}
+    out["sample"] = (f"{steps} oracle steps of the 64^2 x 32 workload (10 onset wavelengths, "
+                     "no trap, dt 5e-5, seeded thermal noise 1e-3)")
+    O.set_workers(cores)
+    return out
 
 
 def mixed_leg(cfg, st0, args, ref, dref) -> dict:
@@ -205,6 +234,53 @@ def mixed_leg(cfg, st0, args, ref, dref) -> dict:
 
 
 CLOSURE_FILE = os.path.join(ROOT, "nhqg_amd", "data", "closure_256x256x128.npz")
+CLOSURE64_FILE = os.path.join(ROOT, "nhqg_amd", "data", "closure_256x256x64.npz")
+
+
+def closure_leg(cfg, st0, args, ref, g_fine) -> dict:
+    """The coarse Nz = 64 step with the learned closure, from the headline
+    state filtered to Nz = 64, timed over the same warmup and steps; judged
+    against the plain 256^3 run at the same step: psi-KE spectrum over the
+    coarse degrees and its top 16, Nu_d, and max_speed of the fine state
+    restricted to the coarse degrees."""
+    from nhqg_amd import Stepper, make_grid
+    from nhqg_amd.closure import (Closure, _filtered_diagnostics, _spectrum_errors,
+                                  filter_state, psi_ke_spectrum)
+
+    cfg_c = cfg.with_updates(Nz=64)
+    out = {}
+    for name, path in (("plain", None), ("closure", CLOSURE64_FILE)):
+        g = make_grid(cfg_c, shells="band")
+        if path:
+            Closure.load(path, nz=64).attach(g)
+        sp = Stepper(g)
+        sp.set_state(filter_state(st0, g_fine, g))
+        sp.run(args.warmup)
+        sp.sync()
+        t0 = time.perf_counter()
+        sp.run_async(args.steps)
+        sp.sync()
+        wall = time.perf_counter() - t0
+        cs, cd = sp.state(), sp.diagnostics()
+        gd = make_grid(cfg_c, shells="band")
+        fd = _filtered_diagnostics(gd, g_fine, ref[0])
+        errs = _spectrum_errors(psi_ke_spectrum(cs.q_hat, g),
+                                psi_ke_spectrum(ref[0].q_hat, g_fine)[:65])
+        out[name] = {"value": args.steps / wall, "unit": "steps/s",
+                     "ms_per_step": 1e3 * wall / args.steps,
+                     "psi_ke_spectrum_rel_err": errs["psi_ke_spectrum"],
+                     "psi_ke_top16_rel_err": errs["psi_ke_top16"],
+                     "Nu_d_rel_err": abs(cd["Nusselt_dealiased"] - ref[1]["Nusselt_dealiased"])
+                     / abs(ref[1]["Nusselt_dealiased"]),
+                     "max_speed_resolved_rel_err": abs(cd["max_speed"] - fd["max_speed"])
+                     / fd["max_speed"]}
+        sp.close()
+        g.close()
+        gd.close()
+    out["config"] = ("256^2 x 64 (the headline state filtered to Nz = 64), closure weights "
+                     + os.path.relpath(CLOSURE64_FILE, ROOT) + "; errors against the plain "
+                     "256^3 run after the same warmup + timed steps")
+    return out
 
 
 def largest_workload(args) -> dict:
@@ -281,6 +357,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=("polar256", "largest"), default="polar256",
                     help="polar256: the 256^2 x 256 headline; largest: 1024^2 x 128 mixed + "
                          "closure")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip the Nz=64 + closure leg")
     ap.add_argument("--mixed-slices", type=int, default=6,
                     help="also time the mixed-precision step with this many int8 slices (4..7) "
                          "and its error against the plain run; 0 skips it")
@@ -326,7 +404,7 @@ def main() -> int:
     wall = time.perf_counter() - t0
     dev_ms = sp.last_ms()
     # state after warmup + timed steps (the profiled step below advances one more)
-    final = sp.state() if args.mixed_slices else None
+    final = sp.state()
     diag = sp.diagnostics()
     prof = sp.ctx.profile_step()
     # plain 1 GiB device copy: practical HBM ceiling and the known-size
@@ -375,6 +453,8 @@ def main() -> int:
                             "calibrated on a 1 GiB copy")
     if args.mixed_slices:
         res["mixed"] = mixed_leg(cfg, st0, args, final, diag)
+    if headline and not args.no_closure:
+        res["nz64_closure"] = closure_leg(cfg, st0, args, (final, diag), g)
     if not args.no_cpu:
         base, par = cpu_baseline(cfg, st0, args.cpu_steps, one)
         res["cpu_baseline"] = base
