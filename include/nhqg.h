@@ -69,6 +69,12 @@ typedef struct {
                         * grid, diagnostics.py:206-208), 1 legacy (under evolve_mean the
                         * -dTheta/dZ w and -eps^2 d<w theta>/dZ terms of explicit_rhs,
                         * solver.py:575-590, and the mean's tau solve, solver.py:746-758) */
+  int32_t npad;        /* horizontal_dealiasing: 0 or nx for the 2/3 rule (products on the
+                        * nx grid, outputs times mask, spectral.py:307-358); 3nx/2 for the
+                        * 3/2 rule (products on the zero-padded npad grid, truncated with the
+                        * factor (npad/nx)^2, spectral.py:18-50, 62-74, 155-253; the SBP
+                        * corrector's horizontal means on that grid).  The 3/2 rule needs the
+                        * full layout, apply_mask 0 and the *_pad operator fields */
   double dt, gamma_imex, ra_sigma, sigma, eps_sq, beta;
 } nhqg_params;
 
@@ -150,6 +156,17 @@ typedef struct {
    * solver.py:231-245): equal to pweight/partner on band-limited layouts */
   const double *pweight_pad;
   const int32_t *partner_pad;
+  /* the padded grid of the 3/2 rule (params.npad = 3nx/2; NULL otherwise):
+   * kidx_of_x_pad (npad): kept row of each padded row x (rows kx = 0..nx/2-1
+   * at x = kx, kx = -nx/2..-1 at x = npad + kx, spectral.py:18-34) or -1;
+   * rows_pad (nkx): padded row of each kept row; twiddle_pad (2*npad):
+   * exp(-2 pi i k / npad); eta_phys_pad, eta_x_phys_pad, eta_y_phys_pad,
+   * sponge_phys_pad (npad x npad): irfft2 of the zero-padded spectra on the
+   * padded grid, without rescaling (grid.py:663-667).  Under the 3/2 rule
+   * these replace eta_phys, eta_x_phys, eta_y_phys and sponge_phys. */
+  const int32_t *kidx_of_x_pad, *rows_pad;
+  const double *twiddle_pad;
+  const double *eta_phys_pad, *eta_x_phys_pad, *eta_y_phys_pad, *sponge_phys_pad;
 } nhqg_operators;
 
 /* Diagnostics vector layout returned by nhqg_diagnostics (length

@@ -40,7 +40,7 @@ class Params(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "nx", "nz", "nkx", "nky", "n_shells", "n_pairs", "substeps", "evolve_mean",
         "advection", "w_neumann", "apply_mask", "mixed_slices", "dense_contractions",
-        "exchange")] + [
+        "exchange", "npad")] + [
         (n, C.c_double) for n in ("dt", "gamma_imex", "ra_sigma", "sigma", "eps_sq", "beta")]
 
 
@@ -59,6 +59,8 @@ _OPS_FIELDS = [
     ("mode_alpha", _dp), ("mode_c", _dp), ("eta_x_phys", _dp), ("eta_y_phys", _dp),
     ("VGZ", _dp), ("mean_solve", _dp), ("mean_diff", _dp), ("mean_flux", _dp),
     ("pweight_pad", _dp), ("partner_pad", _ip),
+    ("kidx_of_x_pad", _ip), ("rows_pad", _ip), ("twiddle_pad", _dp), ("eta_phys_pad", _dp),
+    ("eta_x_phys_pad", _dp), ("eta_y_phys_pad", _dp), ("sponge_phys_pad", _dp),
 ]
 
 
@@ -179,7 +181,10 @@ class Context:
         p.evolve_mean = 1 if grid.evolve_mean else 0
         p.advection = 1 if grid.nonlinear_advection == "jacobian" else 0
         p.w_neumann = 1 if grid.w_bc_top == "neumann" else 0
-        p.apply_mask = 1 if layout.full else 0
+        rule32 = grid.horizontal_dealiasing == "32_rule"
+        # the 2/3 rule masks the outputs of a full layout; the 3/2 rule has no band
+        p.apply_mask = 1 if layout.full and not rule32 else 0
+        p.npad = grid.Npad if rule32 else grid.Nx
         p.mixed_slices = int(getattr(grid, "mixed_slices", 0))
         p.dense_contractions = int(getattr(grid, "dense_contractions", 0))
         p.exchange = 0 if grid.mean_exchange_discretization == "balanced_sbp2_pc" else 1

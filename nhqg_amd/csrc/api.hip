@@ -40,6 +40,14 @@ struct Ops {
   // horizontal-mean weights and partners of the padded 3/2 grid
   double* pw_pad;
   int* partner_pad;
+  // the SBP corrector's horizontal means: the Nx grid's under the 2/3 rule,
+  // the padded grid's under the 3/2 rule (solver.py:873-906)
+  double* pw_corr;
+  int* partner_corr;
+  // tables of the grid the spectral products run on (the Nx grid, or the
+  // padded npad grid under the 3/2 rule): row map, kept rows, twiddles
+  int *fkidx, *frows;
+  double2* ftw;
 };
 
 }  // namespace
@@ -47,6 +55,7 @@ struct Ops {
 struct nhqg_ctx {
   nhqg_params P;
   int Nx = 0, Nk = 0, Nq = 0, Ng = 0, nkx = 0, nky = 0, nm = 0, nm_pad = 0, ld = 0, ldc = 0;
+  int Npad = 0;   // transform size of the spectral products (nx, or 3nx/2)
   int Pred = 0;
   size_t bytes = 0;
   hipStream_t st = nullptr;
@@ -399,7 +408,8 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
        launch_level_scale(n, c->ld, c->mgrad, c->wn, c->thsbp2, c->st));
   }
   FluxArgs a;
-  a.N = c->Nx;
+  a.N = c->Npad;
+  a.out_scale = (double)c->Npad * c->Npad / ((double)c->Nx * c->Nx);
   a.nq = c->Nq;
   a.nkx = c->nkx;
   a.nky = c->nky;
@@ -417,8 +427,8 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   a.sq = (double2*)c->spq;
   a.sw = (double2*)c->spw;
   a.sth = (double2*)c->spth;
-  a.kidx_of_x = o.kidx_of_x;
-  a.rows = o.rows;
+  a.kidx_of_x = o.fkidx;
+  a.rows = o.frows;
   a.kx_row = o.kx_row;
   a.ky_col = o.ky_col;
   a.inv_denom = o.inv_denom;
@@ -428,11 +438,11 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   a.jac = c->P.advection;
   a.eta_x = o.eta_x;
   a.eta_y = o.eta_y;
-  a.tw = o.tw;
+  a.tw = o.ftw;
   a.X5 = c->X5;
   a.Y6 = c->Y6;
   a.fstride = c->fstride;
-  const double F = 16.0 * c->Nq * (double)c->Nx * c->nky;   // one intermediate field
+  const double F = 16.0 * c->Nq * (double)c->Npad * c->nky;   // one intermediate field
   const double Fc = 16.0 * c->Nq * (double)c->nm;            // one compact field
   const int sp = a.sponge_phys ? 3 : 0;
   // intermediates per pass: flux form 5 in / 6 out, Jacobian form 8 in / 3 out
@@ -597,7 +607,7 @@ int corrector_preamble(nhqg_ctx* c, const double* w_in, double total_dt, bool wi
   else
     RUN(gemm_jobs(c, {Job{&o.W2S, w_in, c->wsbp, epi_make(1.0)}}));
   PL(CL_PARSEVAL, 16.0 * n * (double)c->nm,
-     launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->wsbp, o.pw, o.partner, c->partial,
+     launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, c->wsbp, o.pw_corr, o.partner_corr, c->partial,
                      c->st));
   PL(CL_SBP, 0.0, launch_sbp_prepare(n, c->Pred, c->partial, o.d1b, o.lb, 0.5 * mu * kappa * sdt,
                         0.25 * mu * sdt * sdt, c->sbp, c->st));
@@ -644,7 +654,7 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
   PL(CL_SMALL, 0.0, launch_small_axpy(n, thb_in, thb_in, thb_in, 0.0, tb_a, c->st));
   for (int s = 0; s < ns; ++s) {
     PL(CL_PARSEVAL, 32.0 * n * (double)c->nm,
-       launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, cur, o.pw, o.partner, c->partial,
+       launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, cur, o.pw_corr, o.partner_corr, c->partial,
                        c->st));
     PL(CL_SBP, 0.0, launch_sbp_apply(n, c->Pred, c->partial, c->sbp_map, tb_a, tb_b, c->g, c->st));
     std::swap(tb_a, tb_b);
@@ -1010,6 +1020,16 @@ int check_params(const nhqg_params* p, std::string& err) {
     err = "n_shells must be >= 1";
     return NHQG_ERR_ARG;
   }
+  if (p->npad != 0 && p->npad != nx) {
+    if (p->npad != 3 * nx / 2) {
+      err = "npad must be nx (2/3 rule) or 3nx/2 (3/2 rule)";
+      return NHQG_ERR_ARG;
+    }
+    if (p->apply_mask || p->nkx != nx || p->nky != nx / 2 + 1) {
+      err = "the 3/2 rule needs the full mode layout (nkx = nx, nky = nx/2+1) and apply_mask 0";
+      return NHQG_ERR_ARG;
+    }
+  }
   return NHQG_OK;
 }
 
@@ -1023,6 +1043,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   c->nky = p->nky;
   c->nm = p->nkx * p->nky;
   c->nm_pad = (c->nm + 63) / 64 * 64;
+  c->Npad = p->npad ? p->npad : p->nx;
   c->ld = 2 * c->nm_pad;
   c->ldc = c->nm_pad;
   c->Pred = 16;
@@ -1209,19 +1230,41 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
       }
     }
   }
-  RUN(upload(c, &o.eta_phys, h->eta_phys, (size_t)c->Nx * c->Nx));
-  if (p->advection == 1 && h->eta_phys) {
-    if (!h->eta_x_phys || !h->eta_y_phys) {
-      c->err = "the Jacobian form with a trap needs eta_x_phys and eta_y_phys";
+  // trap and sponge on the grid of the spectral products: the Nx grid, or
+  // under the 3/2 rule the padded grid (the *_pad fields; the Nx-grid ones
+  // are not read)
+  const bool pad = c->Npad != c->Nx;
+  const size_t nphys = (size_t)c->Npad * c->Npad;
+  const double* eta = pad ? h->eta_phys_pad : h->eta_phys;
+  const double* eta_x = pad ? h->eta_x_phys_pad : h->eta_x_phys;
+  const double* eta_y = pad ? h->eta_y_phys_pad : h->eta_y_phys;
+  RUN(upload(c, &o.eta_phys, eta, nphys));
+  if (p->advection == 1 && eta) {
+    if (!eta_x || !eta_y) {
+      c->err = pad ? "the Jacobian form with a trap needs eta_x_phys_pad and eta_y_phys_pad"
+                   : "the Jacobian form with a trap needs eta_x_phys and eta_y_phys";
       return NHQG_ERR_ARG;
     }
-    RUN(upload(c, &o.eta_x, h->eta_x_phys, (size_t)c->Nx * c->Nx));
-    RUN(upload(c, &o.eta_y, h->eta_y_phys, (size_t)c->Nx * c->Nx));
+    RUN(upload(c, &o.eta_x, eta_x, nphys));
+    RUN(upload(c, &o.eta_y, eta_y, nphys));
   } else {
     o.eta_x = o.eta_y = nullptr;
   }
-  RUN(upload(c, &o.sponge_phys, h->sponge_phys, (size_t)c->Nx * c->Nx));
+  RUN(upload(c, &o.sponge_phys, pad ? h->sponge_phys_pad : h->sponge_phys, nphys));
   RUN(upload(c, (double**)&o.tw, h->twiddle, (size_t)2 * c->Nx));
+  if (pad) {
+    if (!h->kidx_of_x_pad || !h->rows_pad || !h->twiddle_pad) {
+      c->err = "the 3/2 rule needs kidx_of_x_pad, rows_pad and twiddle_pad";
+      return NHQG_ERR_ARG;
+    }
+    RUN(upload(c, &o.fkidx, h->kidx_of_x_pad, (size_t)c->Npad));
+    RUN(upload(c, &o.frows, h->rows_pad, (size_t)c->nkx));
+    RUN(upload(c, (double**)&o.ftw, h->twiddle_pad, (size_t)2 * c->Npad));
+  } else {
+    o.fkidx = o.kidx_of_x;
+    o.frows = o.rows;
+    o.ftw = o.tw;
+  }
   // mean temperature: the legacy exchange needs all four maps, RK4 under
   // evolve_mean the diffusion
   if (p->evolve_mean && p->exchange == 1 &&
@@ -1246,6 +1289,8 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   } else {
     o.partner_pad = o.partner;
   }
+  o.pw_corr = pad ? o.pw_pad : o.pw;
+  o.partner_corr = pad ? o.partner_pad : o.partner;
   c->h_thb2s.assign(h->thb2s, h->thb2s + (size_t)Nq * Nq);
   c->h_d1b.assign(h->d1_band, h->d1_band + (size_t)Nq * 3);
 
@@ -1289,7 +1334,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(dalloc(c, &c->g, (size_t)Nq));
   // FFT intermediates, one block of 17 fields: flux form 8 in + 9 out,
   // Jacobian form 11 in + 6 out
-  c->fstride = (size_t)Nq * c->Nx * c->nky;
+  c->fstride = (size_t)Nq * c->Npad * c->nky;
   RUN(dalloc(c, &c->X5, c->fstride * 17));
   c->Y6 = c->X5 + c->fstride * (p->advection ? 11 : 8);
   RUN(dalloc(c, &c->staging, (size_t)Nq * c->Nx * c->Nk));
@@ -1377,6 +1422,7 @@ int diagnostics_impl(nhqg_ctx* c, double* out, int n_out) {
     a.qn = (const double2*)c->qn;
     a.wn = (const double2*)c->wn;
     a.thn = (const double2*)c->thn;
+    a.out_scale = 1.0;
     a.kidx_of_x = o.kidx_of_x;
     a.rows = o.rows;
     a.kx_row = o.kx_row;

@@ -134,8 +134,13 @@ static inline void epi_add(Epi& e, double a, const double* X, const double* s = 
 }
 
 // ---- launch wrappers (defined in the .hip files) ----
+// N: the transform length, nx under the 2/3 rule, 3nx/2 (the zero-padded
+// grid) under the 3/2 rule; the tables kidx_of_x (N), rows, tw (N) and the
+// physical fields (N x N) are those of that grid.  out_scale multiplies the
+// outputs of the 3/2 rule ((N/nx)^2); the 2/3 rule multiplies by mask.
 struct FluxArgs {
   int N, nq, nkx, nky, ldc, ng;
+  double out_scale;
   const double2 *qn, *wn, *thn;   // nodal fields (advection)
   const double2 *qc, *wc, *thc;   // coefficient fields (sponge)
   double2 *aq, *aw, *ath;          // advection outputs (nodal spectral)
@@ -150,6 +155,8 @@ struct FluxArgs {
 };
 // pass: 0 all three passes, 1 col_inverse, 2 row products, 3 col_forward
 hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s);
+// the 3/2 rule's passes (a.N = 3P, P = 8..512; fft_pad.hip)
+hipError_t launch_flux_pad(const FluxArgs& a, int pass, hipStream_t s);
 // rowmax: Nq*N row maxima of the speed from psi (a.qn); with a.wn and a.thn
 // set, three more Nq*N sections: max |w|, max |theta|, max |w theta|
 hipError_t launch_speed(const FluxArgs& a, double* rowmax, hipStream_t s);

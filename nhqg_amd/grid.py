@@ -147,6 +147,17 @@ def mask_23(Nx: int) -> np.ndarray:
     return ((np.abs(kxi)[:, None] <= K) & (kyi[None, :] <= K)).astype(np.float64)
 
 
+def zero_pad(f: np.ndarray, Nx: int, Npad: int) -> np.ndarray:
+    """An (..., Nx, Nx/2+1) rfft2 spectrum on the (..., Npad, Npad/2+1) grid:
+    rows kx = 0..Nx/2-1 keep their index, rows kx = -Nx/2..-1 sit at
+    Npad-Nx/2..Npad-1, every other entry zero (nhqg/spectral.py:18-34)."""
+    h = Nx // 2
+    out = np.zeros(f.shape[:-2] + (Npad, Npad // 2 + 1), dtype=np.complex128)
+    out[..., :h, :h + 1] = f[..., :h, :]
+    out[..., Npad - h:, :h + 1] = f[..., h:, :]
+    return out
+
+
 # ---------------------------------------------------------------------------
 # shell inverses (nhqg/grid.py:349-432)
 # ---------------------------------------------------------------------------
@@ -273,6 +284,8 @@ class Grid:
     ksq_idx: np.ndarray
     eta_hat: np.ndarray | None
     sponge_phys: np.ndarray | None
+    sponge_phys_pad: np.ndarray | None
+    Npad: int
     beta: float
     Ra_sigma: float
     sigma: float
@@ -317,7 +330,13 @@ class Grid:
     def evolve_mean(self) -> bool:
         return self.thermal_closure == "evolve_mean"
 
+    @property
+    def rule32(self) -> bool:
+        return self.horizontal_dealiasing == "32_rule"
+
     def layout(self, full: bool = False) -> "ModeLayout":
+        # the 3/2 rule keeps every mode: its only layout is the full one
+        full = full or self.rule32
         key = "full" if full else "band"
         if key not in self._layouts:
             self._layouts[key] = ModeLayout.build(self, full)
@@ -329,6 +348,7 @@ class Grid:
         attached closure is applied to it."""
         from nhqg_amd import _native
 
+        full = full or self.rule32
         ctx = _native.Context(self, self.layout(full))
         if self._closure is not None and not full:
             ctx.set_closure(*self._closure)
@@ -339,6 +359,7 @@ class Grid:
         """The grid's scratch context of one layout, shared by the per-call
         functions (imex_step, rk4_step, compute_diagnostics), which set the
         whole state on every call; created on first use."""
+        full = full or self.rule32
         key = "full" if full else "band"
         if key not in self._contexts:
             self._contexts[key] = self.new_context(full)
@@ -347,6 +368,9 @@ class Grid:
     def attach_closure(self, mlp, n0: int, in_scale: float, out_scale: float) -> None:
         """Attach a trained closure to every band-layout context of this
         grid, present and future (mlp=None detaches)."""
+        if mlp is not None and self.rule32:
+            raise NotImplementedError("the learned closure is built for the 2/3 rule's "
+                                      "band layout")
         self._closure = None if mlp is None else (mlp, int(n0), float(in_scale),
                                                   float(out_scale))
         for ctx in list(self._live):
@@ -378,7 +402,6 @@ def _check_config(cfg: NHQGConfig) -> None:
             raise ValueError(f"Unsupported {name}={v!r}")
     built = {
         "float_dtype": "float64",
-        "horizontal_dealiasing": "23_rule",
         "vertical_dealiasing": "none",
         "q_boundary": "none",
         "imex_scheme": "ars222",
@@ -462,8 +485,10 @@ def make_grid(cfg: NHQGConfig, shells: str = "all") -> Grid:
                 f"energy lies outside the 2/3 band (tolerance 1e-10). Decrease trap_sharpness "
                 f"(tanh width r_star/A_d = {r_star / cfg.trap_sharpness:.3g} vs dx = "
                 f"{L / Nx:.3g}) or increase Nx.")
-        eta_hat = eta_hat * m23
-    sponge = None
+        if cfg.horizontal_dealiasing == "23_rule":
+            eta_hat = eta_hat * m23
+    sponge = sponge_pad = None
+    Npad = 3 * Nx // 2
     if cfg.sponge_rate < 0.0:
         raise ValueError("sponge_rate must be >= 0")
     if cfg.sponge_rate > 0.0:
@@ -484,7 +509,12 @@ def make_grid(cfg: NHQGConfig, shells: str = "all") -> Grid:
                 f"lies outside the 2/3 band (tolerance 1e-10). Decrease sponge_sharpness "
                 f"(tanh width r_s/A_s = {r_s / cfg.sponge_sharpness:.3g} vs dx = "
                 f"{L / Nx:.3g}) or increase Nx.")
-        sponge = np.fft.irfft2(sh * m23, s=(Nx, Nx))
+        if cfg.horizontal_dealiasing == "23_rule":
+            sh = sh * m23
+        sponge = np.fft.irfft2(sh, s=(Nx, Nx))
+        # sigma on the padded grid from its zero-padded spectrum, without the
+        # (Npad/Nx)^2 rescale (the 3/2 rule's sponge product, grid.py:663-667)
+        sponge_pad = np.fft.irfft2(zero_pad(sh, Nx, Npad), s=(Npad, Npad))
 
     gamma_imex = 1.0 - 1.0 / np.sqrt(2.0)
     p = cfg.hyper_order
@@ -504,7 +534,7 @@ def make_grid(cfg: NHQGConfig, shells: str = "all") -> Grid:
         cgl_to_sbp=c2s, sbp_to_cgl=s2c, kx=kx1[:, None], ky=ky1[None, :], ksq=ksq,
         inv_denom=inv_denom, mask_23=m23, diss_rate_q=dq, diss_rate_w=dw, diss_rate_th=dth,
         inv_alpha_q=iaq, inv_alpha_th=iath, ksq_idx=ksq_idx, eta_hat=eta_hat,
-        sponge_phys=sponge, beta=float(cfg.beta), Ra_sigma=cfg.Ra_tilde / cfg.sigma,
+        sponge_phys=sponge, sponge_phys_pad=sponge_pad, Npad=Npad, beta=float(cfg.beta), Ra_sigma=cfg.Ra_tilde / cfg.sigma,
         sigma=float(cfg.sigma), L=float(L), Ld_inv_sq=cfg.Ld_inv_sq, dt=float(cfg.dt),
         gamma_imex=float(gamma_imex), mean_temp_eps_sq=float(cfg.mean_temp_eps_sq),
         thermal_closure=cfg.thermal_closure, q_boundary=cfg.q_boundary, w_bc_top=cfg.w_bc_top,
@@ -598,6 +628,14 @@ class ModeLayout:
             shell_ids=shell_ids, shell_ptr=shell_ptr,
             shell_modes=order.astype(np.int32),
         )
+        if g.rule32:
+            # rows of the zero-padded grid (nhqg/spectral.py:18-34): kx row r
+            # at r for r < Nx/2, at Npad - Nx + r above
+            Np, h = g.Npad, Nx // 2
+            rows_pad = np.where(rows < h, rows, rows - Nx + Np).astype(np.int32)
+            kidx_pad = np.full(Np, -1, dtype=np.int32)
+            kidx_pad[rows_pad] = np.arange(nkx, dtype=np.int32)
+            t.update(rows_pad=rows_pad, kidx_of_x_pad=kidx_pad)
         return ModeLayout(full=full, rows=rows, nky=nky, tables=t)
 
 
@@ -671,13 +709,34 @@ def device_operators(g: Grid, lay: ModeLayout) -> dict:
     if g.w_bc_top == "neumann":
         ops.update(Mtw=Pw @ Sd, Mwt=g.map_w_to_th, Mth2w=g.map_th_to_w)
     Nx = g.Nx
-    ops["eta_phys"] = None if g.eta_hat is None else np.fft.irfft2(g.eta_hat, s=(Nx, Nx))
-    if g.eta_hat is not None and g.nonlinear_advection == "jacobian":
-        # grad(eta) for J(psi, q + eta) (nhqg/solver.py:534-535 adds eta to q
-        # before the Jacobian, nhqg/spectral.py:293-298 differentiates)
-        ops["eta_x_phys"] = np.fft.irfft2(1j * g.kx * g.eta_hat, s=(Nx, Nx))
-        ops["eta_y_phys"] = np.fft.irfft2(1j * g.ky * g.eta_hat, s=(Nx, Nx))
-    ops["sponge_phys"] = g.sponge_phys
+    # the trap and the sponge on the grid of the spectral products: the Nx
+    # grid (2/3 rule) or the zero-padded Npad grid (3/2 rule), in irfft2's
+    # normalisation on that grid, like the products' other factors
+    if g.rule32:
+        Np = g.Npad
+
+        def phys(f):
+            return np.fft.irfft2(zero_pad(f, Nx, Np), s=(Np, Np))
+
+        sfx = "_pad"
+        ops["sponge_phys_pad"] = g.sponge_phys_pad
+        k = np.arange(Np)
+        ops["twiddle_pad"] = np.stack([np.cos(2.0 * np.pi * k / Np),
+                                       -np.sin(2.0 * np.pi * k / Np)], 1)
+    else:
+        def phys(f):
+            return np.fft.irfft2(f, s=(Nx, Nx))
+
+        sfx = ""
+        ops["sponge_phys"] = g.sponge_phys
+    ops["eta_phys" + sfx] = None if g.eta_hat is None else phys(g.eta_hat)
+    if g.eta_hat is not None:
+        if g.nonlinear_advection == "jacobian":
+            # grad(eta) for J(psi, q + eta) (nhqg/solver.py:534-535 adds eta to
+            # q before the Jacobian, nhqg/spectral.py:155-184, 293-298
+            # differentiate)
+            ops["eta_x_phys" + sfx] = phys(1j * g.kx * g.eta_hat)
+            ops["eta_y_phys" + sfx] = phys(1j * g.ky * g.eta_hat)
     # mean temperature (nhqg/solver.py:575-590, 638, 746-758)
     N = g.Nz
     a_m = g.gamma_imex * g.dt * g.mean_temp_eps_sq / g.sigma

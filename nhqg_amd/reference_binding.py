@@ -46,10 +46,10 @@ ARRAY_FIELDS = (
     "Z_sbp", "sbp_H", "sbp_D1", "sbp_L", "cgl_to_sbp", "sbp_to_cgl", "kx", "ky", "ksq",
     "inv_denom", "mask_23", "diss_rate_q", "diss_rate_w", "diss_rate_th", "inv_alpha_q",
     "inv_alpha_th")
-OPTIONAL_FIELDS = ("map_w_to_th", "map_th_to_w", "eta_hat", "sponge_phys")
+OPTIONAL_FIELDS = ("map_w_to_th", "map_th_to_w", "eta_hat", "sponge_phys", "sponge_phys_pad")
 SCALAR_FIELDS = ("beta", "Ra_sigma", "sigma", "L", "Ld_inv_sq", "dt", "gamma_imex",
                  "mean_temp_eps_sq")
-STATIC_FIELDS = ("Nx", "Nk", "Nz", "Nz_gal", "thermal_closure", "q_boundary", "w_bc_top",
+STATIC_FIELDS = ("Nx", "Nk", "Nz", "Nz_gal", "Npad", "thermal_closure", "q_boundary", "w_bc_top",
                  "nonlinear_advection", "vertical_cutoff_n", "imex_scheme",
                  "vertical_dealiasing", "horizontal_dealiasing",
                  "mean_exchange_discretization", "sbp_transfer_mode",
@@ -63,8 +63,10 @@ def enabled() -> bool:
 
 def unsupported_reason(ref) -> str | None:
     """Why the device path does not cover this reference Grid, or None."""
-    want = {"horizontal_dealiasing": "23_rule", "vertical_dealiasing": "none",
-            "q_boundary": "none", "imex_scheme": "ars222", "sbp_transfer_mode": "interp"}
+    want = {"vertical_dealiasing": "none", "q_boundary": "none", "imex_scheme": "ars222",
+            "sbp_transfer_mode": "interp"}
+    if ref.horizontal_dealiasing not in ("23_rule", "32_rule"):
+        return f"horizontal_dealiasing={ref.horizontal_dealiasing!r}"
     for name, v in want.items():
         if getattr(ref, name) != v:
             return f"{name}={getattr(ref, name)!r}"
@@ -74,6 +76,8 @@ def unsupported_reason(ref) -> str | None:
         return "vertical_cutoff_n"
     if ref.Nx < 16 or ref.Nx > 1024 or ref.Nx & (ref.Nx - 1):
         return f"Nx={ref.Nx}"
+    if ref.Npad != 3 * ref.Nx // 2:
+        return f"Npad={ref.Npad}"
     if not 3 <= ref.Nz <= 256:
         return f"Nz={ref.Nz}"
     if np.asarray(ref.dt).dtype != np.float64:

@@ -36,7 +36,10 @@ class State(NamedTuple):
 
 
 def is_band_limited(state: State, grid: Grid) -> bool:
-    """True when q, w, theta are zero outside the 2/3 band."""
+    """True when q, w, theta are zero outside the 2/3 band (always, under
+    the 3/2 rule, which keeps every mode)."""
+    if grid.rule32:
+        return True
     out = grid.mask_23 == 0.0
     for f in (state.q_hat, state.w_hat, state.th_hat):
         if np.any(np.asarray(f)[:, out] != 0):
@@ -63,7 +66,10 @@ def needs_full_layout(state: State, grid: Grid) -> bool:
     Hermitian self-conjugate columns; any other state takes its next step in
     the full layout, where out-of-band and anti-Hermitian content enter the
     products and means exactly as in the reference and the step's final
-    sanitation removes them."""
+    sanitation removes them.  Under the 3/2 rule every context holds the
+    full layout."""
+    if grid.rule32:
+        return False
     return not (is_band_limited(state, grid) and is_hermitian(state, grid))
 
 
@@ -96,8 +102,8 @@ def rk4_step(state: State, grid: Grid) -> State:
 
 
 def sanitize_state(state: State, grid: Grid) -> State:
-    """Hermitian projection of the self-conjugate rfft2 columns and 2/3 band
-    limit (nhqg/solver.py:107-146), on the host."""
+    """Hermitian projection of the self-conjugate rfft2 columns and, under
+    the 2/3 rule, the band limit (nhqg/solver.py:107-146), on the host."""
     Nx = grid.Nx
     neg = (-np.arange(Nx)) % Nx
     out = []
@@ -107,7 +113,7 @@ def sanitize_state(state: State, grid: Grid) -> State:
         for c in cols:
             col = f[:, :, c].copy()
             f[:, :, c] = 0.5 * (col + np.conj(col[:, neg]))
-        out.append(f * grid.mask_23[None])
+        out.append(f if grid.rule32 else f * grid.mask_23[None])
     return State(out[0], out[1], out[2], np.array(state.th_bar, dtype=np.float64))
 
 

@@ -26,6 +26,9 @@ CONFIGS = {
                                              mean_exchange_discretization="legacy",
                                              hyper_order=2, nu_q=1e-3, nu_w=2e-3,
                                              nu_theta=3e-3, drag=0.05),
+    # the reference's default dealiasing: products on the 3Nx/2 grid
+    "rule32_trap_sponge": lambda: cases.trap_sponge_cfg(horizontal_dealiasing="32_rule",
+                                                        nonlinear_advection="jacobian"),
 }
 
 
@@ -66,9 +69,12 @@ def test_binding_recovers_rates_from_oracle_arrays():
 
 
 def test_binding_rejects_branches_not_built():
+    # the reference defaults (3/2 rule, Jacobian form, legacy exchange) in FP64
     ref = from_oracle(NHQGConfig(Nx=32, Nz=8, L=20.0, dt=1e-3, float_dtype="float64"))
-    assert not RB.supported(ref)                       # reference defaults: 3/2 rule
-    assert "horizontal_dealiasing" in RB.unsupported_reason(ref)
+    assert RB.supported(ref)
+    ref = ref._replace(sbp_transfer_mode="mass_adjoint")
+    assert not RB.supported(ref)
+    assert "sbp_transfer_mode" in RB.unsupported_reason(ref)
     with pytest.raises(NotImplementedError):
         RB.grid_from_reference(ref)
 
@@ -104,7 +110,7 @@ def test_binding_step_is_bit_identical_to_the_package_step(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["fixture", "trap_sponge", "neumann"])
+@pytest.mark.parametrize("name", ["fixture", "trap_sponge", "neumann", "rule32_trap_sponge"])
 def test_binding_on_oracle_operators_matches_oracle(name):
     cfg = CONFIGS[name]()
     print("binding vs oracle", name, flush=True)

@@ -129,7 +129,7 @@ def test_config_guards():
     with pytest.raises(ValueError, match="Unsupported w_bc_top"):
         make_grid(cases.fixture_cfg(w_bc_top="open"))
     with pytest.raises(NotImplementedError):
-        make_grid(cases.fixture_cfg(horizontal_dealiasing="32_rule"))
+        make_grid(cases.fixture_cfg(vertical_dealiasing="cheb_3o2"))
     with pytest.raises(NotImplementedError):
         make_grid(cases.fixture_cfg(float_dtype="float32"))
     with pytest.raises(ValueError, match="mutually exclusive"):
@@ -192,3 +192,43 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one MI355X" in r.stderr
+
+
+def test_rule32_grid_tables_match_the_padding():
+    """3/2 rule: the padded-grid row tables place every kept row where the
+    oracle's zero_pad (nhqg/spectral.py:18-34) puts it, the trap is not
+    masked, sigma on the padded grid is the oracle's, and the device's
+    physical trap / gradient fields are irfft2 of the zero-padded spectra."""
+    from nhqg_amd.grid import device_operators, zero_pad
+
+    for adv in ("flux", "jacobian"):
+        cfg = cases.trap_sponge_cfg(horizontal_dealiasing="32_rule", nonlinear_advection=adv)
+        g = make_grid(cfg)
+        o = O.build_operators(cfg, with_shells=False)
+        lay = g.layout(False)
+        assert lay.full and g.Npad == 3 * cfg.Nx // 2
+        t = lay.tables
+        Nx, Np = cfg.Nx, g.Npad
+        # a field that is 1 + its row index on every kept mode lands on the
+        # padded rows exactly as the oracle's zero_pad places it
+        f = (1.0 + np.arange(Nx))[:, None] * np.ones((1, Nx // 2 + 1))
+        zp = O.zero_pad(f[None].astype(complex), Nx, Np)[0][:, :Nx // 2 + 1].real
+        for x in range(Np):
+            i = t["kidx_of_x_pad"][x]
+            assert (zp[x] == 0).all() if i < 0 else (zp[x] == 1.0 + i).all()
+        assert np.array_equal(t["kidx_of_x_pad"][t["rows_pad"]], np.arange(Nx))
+        assert np.array_equal(g.eta_hat, o.eta_hat)
+        assert np.abs(g.eta_hat * (1 - g.mask_23)).max() > 0     # unmasked
+        assert np.array_equal(g.sponge_phys_pad, o.sponge_phys_pad)
+        ops = device_operators(g, lay)
+        assert ops["eta_phys_pad"].shape == (Np, Np) and ops.get("eta_phys") is None
+        want = np.fft.irfft2(zero_pad(o.eta_hat, Nx, Np), s=(Np, Np))
+        assert np.abs(ops["eta_phys_pad"] - want).max() <= 1e-15 * np.abs(want).max()
+        assert np.array_equal(zero_pad(o.eta_hat, Nx, Np), O.zero_pad(o.eta_hat, Nx, Np))
+        if adv == "jacobian":
+            gx = np.fft.irfft2(O.zero_pad(1j * o.kx * o.eta_hat, Nx, Np), s=(Np, Np))
+            assert np.abs(ops["eta_x_phys_pad"] - gx).max() <= 1e-15 * np.abs(gx).max()
+        k = np.arange(Np)
+        tw = ops["twiddle_pad"]
+        assert np.abs(tw[:, 0] + 1j * tw[:, 1] - np.exp(-2j * np.pi * k / Np)).max() < 1e-15
+

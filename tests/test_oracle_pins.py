@@ -376,6 +376,92 @@ def test_flux32_and_sponge32_are_exact_convolutions():
             assert abs(got_s[i, ky] - ws) < 1e-12 * max(1.0, abs(ws)), (i, ky)
 
 
+def _padded_components(h, N):
+    """Fourier components (signed wavenumbers) of M^2 irfft2(zero_pad(h),
+    (M, M)) from numpy's documented irfft2 semantics: the inverse along x,
+    then along y a half spectrum whose ky = 0 entry counts by its real part
+    only, every other stored column ky also by its conjugate at (-kx, -ky).
+    Row N/2 of h is kx = -N/2, column N/2 an ordinary column of the M grid."""
+    comp = {}
+
+    def add(k, v):
+        comp[k] = comp.get(k, 0.0) + v
+
+    for i in range(N):
+        kx = i if i < N // 2 else i - N
+        for ky in range(N // 2 + 1):
+            v = h[i, ky]
+            if v == 0:
+                continue
+            if ky == 0:
+                add((kx, 0), v / 2)
+                add((-kx, 0), np.conj(v) / 2)
+            else:
+                add((kx, ky), v)
+                add((-kx, -ky), np.conj(v))
+    return comp
+
+
+def _aliased_product(a, b, M):
+    """Components of the product of two fields on the M grid: sums over
+    k1 + k2, wavenumbers folded into [-M/2, M/2) (the aliasing of the grid)."""
+    out = {}
+    for (ax, ay), va in a.items():
+        for (bx, by), vb in b.items():
+            k = ((ax + bx + M // 2) % M - M // 2, (ay + by + M // 2) % M - M // 2)
+            out[k] = out.get(k, 0.0) + va * vb
+    return out
+
+
+@pytest.mark.parametrize("modulus", ["3N/2", "2N"])
+def test_rule32_products_alias_nyquist_content_on_the_3n2_grid(modulus):
+    """With content on the kx = -N/2 row and the ky = N/2 column, the 3/2
+    rule's products (flux, Jacobian, sponge) keep the alias of the products
+    of two such components that the 3N/2 grid folds onto kept modes: the
+    oracle equals the modular convolution with M = 3N/2 to roundoff, and
+    differs from the one of a 2N grid (which has no such alias)."""
+    N, L = 8, 5.0
+    o = O.build_operators(_cfg(Nx=N, Nz=2, L=L, horizontal_dealiasing="32_rule",
+                               nonlinear_advection="flux"), with_shells=False)
+    rng = np.random.default_rng(11)
+    k0 = 2 * np.pi / L
+    ph = np.fft.rfft2(rng.standard_normal((N, N))) + 0.5 * (
+        rng.standard_normal((N, N // 2 + 1)) + 1j * rng.standard_normal((N, N // 2 + 1)))
+    fh = np.fft.rfft2(rng.standard_normal((N, N)))
+    fh[N // 2, 1:] += 1.0 - 2.0j                     # Nyquist row and column content
+    fh[2:5, N // 2] += 0.7 + 0.3j
+    M = 3 * N // 2 if modulus == "3N/2" else 2 * N
+    kxr = np.array([i if i < N // 2 else i - N for i in range(N)])[:, None] * k0
+    kyr = np.arange(N // 2 + 1)[None, :] * k0
+    C = lambda h: _padded_components(h, N)
+    cu = _aliased_product(C(-1j * kyr * ph), C(fh), M)
+    cv = _aliased_product(C(1j * kxr * ph), C(fh), M)
+    cj = _aliased_product(C(1j * kxr * ph), C(1j * kyr * fh), M)
+    cj2 = _aliased_product(C(1j * kyr * ph), C(1j * kxr * fh), M)
+    sig = np.fft.rfft2(rng.standard_normal((N, N)))
+    sig[N // 2, 2] += 0.4
+    o.sponge_phys_pad = np.fft.irfft2(O.zero_pad(sig[None], N, o.Npad)[0], s=(o.Npad, o.Npad))
+    cs = _aliased_product(C(sig), C(fh), M)
+    (F,) = O.flux32(ph[None], [fh[None]], o)
+    (J,) = O.jacobian32(ph[None], [fh[None]], o)
+    S = O.sponge32(fh[None], o)[0]
+    worst = 0.0
+    for i in range(N):
+        kx = i if i < N // 2 else i - N
+        for ky in range(N // 2 + 1):
+            k = (kx, ky)
+            wf = (1j * kx * k0 * cu.get(k, 0) + 1j * ky * k0 * cv.get(k, 0)) / N ** 2
+            wj = (cj.get(k, 0) - cj2.get(k, 0)) / N ** 2
+            ws = cs.get(k, 0) / N ** 2
+            worst = max(worst, abs(F[0, i, ky] - wf) / max(1.0, abs(wf)),
+                        abs(J[0, i, ky] - wj) / max(1.0, abs(wj)),
+                        abs(S[i, ky] - ws) / max(1.0, abs(ws)))
+    if modulus == "3N/2":
+        assert worst < 1e-12
+    else:
+        assert worst > 1e-3
+
+
 # ---------------------------------------------------------------------------
 # SBP substep: energy balance
 # ---------------------------------------------------------------------------
