@@ -283,6 +283,84 @@ def closure_leg(cfg, st0, args, ref, g_fine) -> dict:
     return out
 
 
+# The reference's own published step times: its benchmark harness
+# (scripts/benchmark.py:22-74: 3/2 rule, Jacobian form, legacy exchange,
+# fixed conduction, hyper-4, dt 5e-4, L 20) on a V100-SXM2-32GB (CLAUDE.md:
+# 495-505), and the 256^2 x 256 production branch on an H200
+# (efficiency_review.md:3-6).
+V100_TABLE_MS = {(64, 8): 0.76, (128, 16): 2.88, (256, 32): 24.2, (512, 32): 98.5,
+                 (512, 64): 214.8, (1024, 64): 967.2}
+H200_256_MS = 150.0
+
+
+def _time_steps(g, st, warmup: int, steps: int) -> tuple:
+    """(ms per step of `steps` graph replays, ms per step with a host
+    synchronisation after every step as the reference's harness times)."""
+    from nhqg_amd import Stepper
+
+    sp = Stepper(g)
+    sp.set_state(st)
+    sp.run(warmup)
+    sp.sync()
+    t0 = time.perf_counter()
+    sp.run_async(steps)
+    sp.sync()
+    thr = 1e3 * (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(min(steps, 20)):
+        sp.run(1)
+    syn = 1e3 * (time.perf_counter() - t0) / min(steps, 20)
+    d = sp.diagnostics()
+    sp.close()
+    return thr, syn, d
+
+
+def published_configs_leg(args) -> dict:
+    """The configurations behind the reference's published step times, run
+    here as plain FP64 steps: the 3/2-rule 256^2 x 256 production branch
+    (H200: 150 ms/step) and the benchmark harness table (V100)."""
+    import numpy as np
+
+    from nhqg_amd import NHQGConfig, make_grid
+    from nhqg_amd.init import polar_config_256, polar_state_256, thermal_noise_state
+
+    out = {}
+    cfg = polar_config_256(horizontal_dealiasing="32_rule", dt=5e-5)
+    g = make_grid(cfg, shells="band")
+    thr, syn, d = _time_steps(g, polar_state_256(g, seed=0), 5, 50)
+    out["h200_production_256x256x256_rule32"] = {
+        "config": "balanced_sbp2_pc sub4, flux, evolve_mean, 3/2 rule (products on the "
+                  "384^2 padded grid), polar trap + sponge, Nx=Ny=256, Nz=256, dt=5e-5",
+        "ms_per_step": thr, "ms_per_step_synced": syn,
+        "published_ms_per_step": H200_256_MS, "published_hardware": "1x H200 (JAX FP64)",
+        "speedup_vs_published": H200_256_MS / syn,
+        "finite": bool(np.isfinite(d["KE_tot"])),
+        "source": "reference efficiency_review.md:3-6"}
+    g.close()
+    rows = []
+    for (nx, nz), pub in V100_TABLE_MS.items():
+        dt = 5e-4
+        kmax = np.pi * nx / 20.0
+        c = NHQGConfig(Nx=nx, Nz=nz, L=20.0, Ra_tilde=100.0, sigma=1.0, dt=dt,
+                       nu_q=5.0 * dt / kmax ** 8, hyper_order=4, nu_w=1e-3, nu_theta=1e-3,
+                       float_dtype="float64")
+        g = make_grid(c, shells="band")
+        thr, syn, d = _time_steps(g, thermal_noise_state(g, seed=0, amplitude=1e-3), 3, 30)
+        rows.append({"nx": nx, "nz": nz, "ms_per_step": thr, "ms_per_step_synced": syn,
+                     "published_v100_ms": pub, "speedup_vs_published": pub / syn,
+                     "finite": bool(np.isfinite(d["KE_tot"]))})
+        g.close()
+    out["benchmark_py_table"] = {
+        "config": "scripts/benchmark.py: 3/2 rule, Jacobian form, legacy exchange, fixed "
+                  "conduction, hyper-4 (nu_q = 5 dt / k_max^8), nu_w = nu_theta = 1e-3, "
+                  "dt 5e-4, L 20, seeded thermal noise 1e-3",
+        "rows": rows, "published_hardware": "V100-SXM2-32GB (JAX 0.9.1 FP64)",
+        "source": "reference CLAUDE.md:495-505"}
+    out["note"] = ("speedups use the synchronised per-step time, as the reference harness "
+                   "blocks on every step; published figures are the reference authors' own")
+    return out
+
+
 def largest_workload(args) -> dict:
     """Largest single-GPU case: Nx=Ny=1024, Nz=128, polar trap + sponge,
     dt=1e-5, the mixed-precision step with the learned closure, from a
@@ -354,11 +432,16 @@ def main() -> int:
     ap.add_argument("--nz", type=int, default=256, help="override Nz (other workloads)")
     ap.add_argument("--nx", type=int, default=256, help="override Nx = Ny (other workloads)")
     ap.add_argument("--dt", type=float, default=2.5e-5, help="override dt (other workloads)")
+    ap.add_argument("--dealiasing", choices=("23_rule", "32_rule"), default="23_rule",
+                    help="horizontal_dealiasing of the polar256 workload (the headline is the "
+                         "2/3 rule; 32_rule runs the products on the 3Nx/2 grid)")
     ap.add_argument("--workload", choices=("polar256", "largest"), default="polar256",
                     help="polar256: the 256^2 x 256 headline; largest: 1024^2 x 128 mixed + "
                          "closure")
     ap.add_argument("--no-closure", action="store_true",
                     help="skip the Nz=64 + closure leg")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the legs that rerun the reference's published configurations")
     ap.add_argument("--mixed-slices", type=int, default=6,
                     help="also time the mixed-precision step with this many int8 slices (4..7) "
                          "and its error against the plain run; 0 skips it")
@@ -379,8 +462,9 @@ def main() -> int:
         print(json.dumps(res), flush=True)
         return 0 if np.isfinite(res["value"]) and res["finite"] else 1
 
-    cfg = polar_config_256(Nx=args.nx, Nz=args.nz, dt=args.dt)
-    headline = (args.nx, args.nz, args.dt) == (256, 256, 2.5e-5)
+    cfg = polar_config_256(Nx=args.nx, Nz=args.nz, dt=args.dt,
+                           horizontal_dealiasing=args.dealiasing)
+    headline = (args.nx, args.nz, args.dt, args.dealiasing) == (256, 256, 2.5e-5, "23_rule")
     t0 = time.perf_counter()
     g = make_grid(cfg, shells="band")
     t_grid = time.perf_counter() - t0
@@ -428,7 +512,8 @@ def main() -> int:
         "config": {
             "workload": f"plain FP64 ARS222 balanced_sbp2_pc step, Nx=Ny={g.Nx}, Nz={g.Nz}, "
                         "Ra=100, Pr=1, L=24 onset wavelengths, polar trap gamma=2.2e-3 + "
-                        f"sponge, flux advection, 2/3 rule, dt={cfg.dt:g}",
+                        f"sponge, flux advection, "
+                        f"{'2/3' if args.dealiasing == '23_rule' else '3/2'} rule, dt={cfg.dt:g}",
             "nx": g.Nx, "nz": g.Nz, "band_modes": lay.nm, "stored_modes": g.Nx * g.Nk,
             "parallelism": "1 GPU",
         },
@@ -455,6 +540,8 @@ def main() -> int:
         res["mixed"] = mixed_leg(cfg, st0, args, final, diag)
     if headline and not args.no_closure:
         res["nz64_closure"] = closure_leg(cfg, st0, args, (final, diag), g)
+    if headline and not args.no_published:
+        res["published_configs"] = published_configs_leg(args)
     if not args.no_cpu:
         base, par = cpu_baseline(cfg, st0, args.cpu_steps, one)
         res["cpu_baseline"] = base
