@@ -493,7 +493,7 @@ def main() -> int:
     prof = sp.ctx.profile_step()
     # plain 1 GiB device copy: practical HBM ceiling and the known-size
     # dispatch that calibrates the FETCH_SIZE / WRITE_SIZE counters
-    copy_gbps = sp.ctx.stream_copy(1 << 30, reps=5)
+    copy_gbps, memcpy_gbps = sp.ctx.stream_copy(1 << 30, reps=5)
 
     res = {
         "metric": METRIC,
@@ -519,6 +519,7 @@ def main() -> int:
         },
         "roofline": None,
         "hbm_copy_gbps": copy_gbps,
+        "hbm_memcpy_gbps": memcpy_gbps,
         "kernels": {k: {"ms_per_step": v[0], "launches": v[1]} for k, v in prof.items()},
         "launches_per_step": int(sum(v[1] for v in prof.values())),
         "setup_s": {"make_grid_host": t_grid,

@@ -276,9 +276,11 @@ const char *nhqg_timing_name(int32_t i);
 
 /* Measurement utility with no counterpart in the reference: a plain
  * streaming copy of `bytes` (read + write), repeated `reps` times on the
- * context stream; returns the achieved GB/s (read + write bytes).  Used to
- * calibrate HBM traffic counters and as a practical bandwidth ceiling. */
-int nhqg_stream_copy(nhqg_ctx *ctx, uint64_t bytes, int32_t reps, double *gbps);
+ * context stream.  gbps[0]: the library's copy kernel (stream_copy_kernel,
+ * the known-size dispatch that calibrates HBM traffic counters); gbps[1]:
+ * hipMemcpyAsync device-to-device over the same buffers.  Both count read +
+ * write bytes; the larger is the practical bandwidth ceiling. */
+int nhqg_stream_copy(nhqg_ctx *ctx, uint64_t bytes, int32_t reps, double gbps[2]);
 
 /* Copy the inverse of the s-th shell factor held by the context
  * (imex_inv[s] of nhqg/grid.py:430, Ng x Ng row-major) to out. */

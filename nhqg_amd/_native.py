@@ -304,11 +304,12 @@ class Context:
         return res
 
     def stream_copy(self, nbytes, reps=10):
-        """Achieved GB/s (read + write) of a plain device copy of nbytes."""
-        v = C.c_double()
-        self._check(self.lib.nhqg_stream_copy(self.h, int(nbytes), int(reps), C.byref(v)),
+        """(kernel, hipMemcpyAsync) GB/s, read + write bytes, of a plain
+        device-to-device copy of nbytes."""
+        v = (C.c_double * 2)()
+        self._check(self.lib.nhqg_stream_copy(self.h, int(nbytes), int(reps), v),
                     "nhqg_stream_copy")
-        return v.value
+        return float(v[0]), float(v[1])
 
     def shell_inverse(self, s: int) -> np.ndarray:
         """inv(A'(k)) of the s-th touched shell as held on the device."""

