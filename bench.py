@@ -88,15 +88,20 @@ def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
         traffic=profiled_traffic("gemm_f64_mfma") if full_size else None)
     t = sum(prof[k][0] for k in ("flux_col_inverse", "flux_row_products", "flux_col_forward"))
     nrhs = prof["flux_col_inverse"][1]
-    alg = 6.0 * 16.0 * nq * nm * nrhs          # q, w, theta nodal in + 3 tendencies out
+    # q, w, theta nodal in + 3 tendencies out; with the sponge also the
+    # coefficient levels of q, w, theta in and their 3 sponge terms out
+    sponge = g.sponge_phys is not None or g.sponge_phys_pad is not None
+    alg = 16.0 * nm * nrhs * (6.0 * nq + (2.0 * (nq + 2 * ng) if sponge else 0.0))
     ach = alg / (t * 1e-3) / 1e9
     out["flux"] = dict(
         kernel="col_inverse_flux + row_flux2 + col_forward_flux (one FFT-product pipeline per RHS)",
         launches_per_step=3 * nrhs, ms_per_step=t, avg_launch_us=1e3 * t / (3 * nrhs), bound="hbm",
         achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
         work_per_launch=alg / (3 * nrhs),
-        work="6 fields (q, w, theta nodal in; 3 tendencies out) x (Nz+1) levels x band modes x 16 B "
-             "per RHS",
+        work="per RHS: q, w, theta nodal in and their 3 tendencies out, (Nz+1) levels each"
+             + ("; with the sponge also the coefficient levels of q (Nz+1), w and theta (Nz-1 "
+                "each) in and their 3 sponge terms out" if sponge else "")
+             + "; x band modes x 16 B",
         traffic=profiled_traffic("flux") if full_size else None)
     ms, n, work = prof.get("gemm_i8_mixed", (0.0, 0, 0.0))
     if n > 0:

@@ -7,13 +7,14 @@ the device tables of the MI355X step: fused vertical operators, the
 band-compact mode layout, shell groupings and FFT twiddles.  The device
 context is created lazily on first use.
 
-Supported branch (anything else raises): float64, ``horizontal_dealiasing=
-'23_rule'``, ``vertical_dealiasing='none'``, ``q_boundary='none'``,
-``nonlinear_advection`` either ``'flux'`` or ``'jacobian'``, ``imex_scheme='ars222'``,
-``sbp_transfer_mode='interp'``, ``thermal_closure`` either
-``'fixed_conduction'`` or ``'evolve_mean'`` with
-``mean_exchange_discretization='balanced_sbp2_pc'``, either ``w_bc_top``,
-power-of-two ``Nx`` in [16, 1024], ``Nz`` <= 256.
+Supported branch (anything else raises): float64, ``horizontal_dealiasing``
+either ``'23_rule'`` (band-compact layout) or ``'32_rule'`` (full layout, products
+on the zero-padded 3Nx/2 grid), ``vertical_dealiasing='none'``,
+``q_boundary='none'``, ``nonlinear_advection`` either ``'flux'`` or
+``'jacobian'``, ``imex_scheme='ars222'``, ``sbp_transfer_mode='interp'``,
+``thermal_closure`` either ``'fixed_conduction'`` or ``'evolve_mean'``,
+``mean_exchange_discretization`` either ``'balanced_sbp2_pc'`` or ``'legacy'``,
+either ``w_bc_top``, power-of-two ``Nx`` in [16, 1024], ``Nz`` <= 256.
 """
 
 from __future__ import annotations
