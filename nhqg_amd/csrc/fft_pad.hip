@@ -20,12 +20,22 @@
 //   col_forward_pad : per (level, ky): forward DFT along x, the Nx kept
 //                     rows, i kx, times (M/Nx)^2                     -> A
 //
-// The M-point transform (team_fft3) is one decimation-in-frequency radix-3
-// stage over the team's LDS buffer followed by three P-point radix-4
+// An M-point transform is one radix-3 step and three P-point radix-4
 // transforms (team_fft), each run by a group of P/4 of the team's 3P/4
-// threads; its bins come out in decimated order (bin 3m + p at sub-buffer
-// p, position m).  Every barrier of a team transform is a block barrier, so
-// all threads of a workgroup take every transform, idle teams included.
+// threads:
+//   inverse transforms (team_fft3, decimation in frequency): the radix-3
+//     stage over the team's LDS buffer, then the sub-transforms; the bins
+//     come out in decimated order (bin 3m + p at sub-buffer p, position m);
+//   forward transforms (decimation in time): input x[3n + r] goes to
+//     sub-buffer r, and each output bin k' + pP = sum_r w3^(rp) w^(rk') Y_r[k']
+//     is combined when it is read (dit_bin), only for the rows and ky that
+//     are kept (the column pass needs 2P of the 3P rows).
+// Building the inverse transforms' radix-3 inputs straight from global memory
+// instead (each group loading the inputs of its own sub-transform) needed
+// 2-3x the load instructions and more registers: it took the row pass from
+// 3.1 to 5.8 ms per step at 256^3.  Every barrier that spans the groups of a
+// team is a block barrier, which every thread of a workgroup reaches, idle
+// teams included.
 #include "fft_team.h"
 
 // Columns per column-pass workgroup (C) and its threads (C teams of 3P/4):
@@ -42,7 +52,8 @@ struct RowBlock3 {
   static constexpr int v = (3 * P / 4 > 192) ? 3 * P / 4 : 192;
 };
 
-// Position in a team buffer of bin x of team_fft3's (decimated) output.
+// Position in a team buffer of bin x of an inverse (decimation-in-frequency)
+// transform's output.
 template <int P>
 NHQG_DEV int dec3(int x) {
   return (x % 3) * P + swz(x / 3);
@@ -54,6 +65,42 @@ template <int P>
 NHQG_DEV void load_twiddles3(double2* tw3, double2* twP, const double2* twg) {
   for (int i = threadIdx.x; i < 2 * P; i += blockDim.x) tw3[i] = twg[i];
   load_twiddles<P>(twP, twg, 3);
+}
+
+// x0 + w3^p x1 + w3^2p x2 with w3 = exp(-2 pi i / 3), or its conjugate (INV)
+template <bool INV>
+NHQG_DEV double2 radix3(double2 x0, double2 x1, double2 x2, int p) {
+  constexpr double H = 0.86602540378443864676;  // sqrt(3) / 2
+  const double2 s = cadd(x1, x2), d = csub(x1, x2);
+  const double2 m = make_double2(x0.x - 0.5 * s.x, x0.y - 0.5 * s.y);
+  const double2 hd = make_double2(-H * d.y, H * d.x);  // i (sqrt(3)/2) d
+  const double2 r = ((p == 1) != INV) ? csub(m, hd) : cadd(m, hd);
+  return p == 0 ? cadd(x0, s) : r;
+}
+
+// The P-point transform of the caller's group (threads t / (P/4) of a 3P/4
+// team) on sub-buffer r of the team buffers a, b (natural order in, natural
+// order out); returns the team buffer, a or b, that holds every group's
+// result.  Opens with the group's barrier: the group's threads wrote their
+// sub-buffer (a caller whose sub-buffers were written across groups takes a
+// block barrier first).
+template <int P, bool INV>
+NHQG_DEV double2* sub_fft3(double2* a, double2* b, int t, const double2* twP) {
+  constexpr int G = P / 4;
+  const int r = t / G;
+  team_sync<G>();
+  double2* res = team_fft<P, INV>(a + r * P, b + r * P, t % G, twP);
+  return res == a + r * P ? a : b;
+}
+
+// Forward-transform bin k from the three sub-transform results in buffer z
+// (decimation in time): sum_r w3^(rp) w^(r k') Y_r[k'], k = k' + pP.
+template <int P>
+NHQG_DEV double2 dit_bin(const double2* z, const double2* tw3, int k) {
+  const int kp = k % P, p = k / P;
+  const double2 a1 = cmul(z[P + swz(kp)], tw3[kp]);
+  const double2 a2 = cmul(z[2 * P + swz(kp)], tw3[2 * kp]);
+  return radix3<false>(z[swz(kp)], a1, a2, p);
 }
 
 // Unnormalised DFT of M = 3P points by a team of 3P/4 threads (index t):
@@ -128,15 +175,19 @@ NHQG_DEV void fwd3_pair(double2* ba, double2* bb, int t, const double2* tw3, con
                         double2 (&B)[2]) {
   constexpr int M = 3 * P, T = 3 * P / 4;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) ba[swz(t + j * T)] = make_double2(a[j], b ? b[j] : 0.0);
+  for (int j = 0; j < 4; ++j) {
+    const int y = t + j * T;
+    ba[(y % 3) * P + swz(y / 3)] = make_double2(a[j], b ? b[j] : 0.0);
+  }
   __syncthreads();
-  const double2* r = team_fft3<P, false>(ba, bb, t, tw3, twP);
+  const double2* z = sub_fft3<P, false>(ba, bb, t, twP);
+  __syncthreads();
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const int ky = t + s * T;
     if (ky < nky) {
-      const double2 zk = r[dec3<P>(ky)];
-      const double2 zm = r[dec3<P>((M - ky) % M)];
+      const double2 zk = dit_bin<P>(z, tw3, ky);
+      const double2 zm = dit_bin<P>(z, tw3, (M - ky) % M);
       A[s] = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
       B[s] = make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
     } else {
@@ -444,16 +495,19 @@ col_forward_pad(int nkx, int nky, int ldc, int ng, const double2* __restrict__ Y
       const int idx = tid + it * BS;
       const int cc = idx % TEAMS, x = idx / TEAMS;
       const int kyc = ky0 + cc;
-      buf[0][cc][swz(x)] = (kyc < nky) ? src[(size_t)x * nky + kyc] : make_double2(0.0, 0.0);
+      // decimation in time: row x to sub-buffer x % 3, position x / 3
+      buf[0][cc][(x % 3) * P + swz(x / 3)] =
+          (kyc < nky) ? src[(size_t)x * nky + kyc] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    const double2* r = team_fft3<P, false>(buf[0][team], buf[1][team], t, tw3, twP);
+    const double2* zb = sub_fft3<P, false>(buf[0][team], buf[1][team], t, twP);
+    __syncthreads();   // a kept row's bin combines all three groups' results
     double2* const dout = f == 0 ? out_q : f == 1 ? out_w : out_th;
     double2* const dsp = f == 0 ? sp_q : f == 1 ? sp_w : sp_th;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (!in[j]) continue;
-      const double2 z = r[dec3<P>(rowv[j])];
+      const double2 z = dit_bin<P>(zb, tw3, rowv[j]);
       const size_t o = (size_t)l * ldc + ky * nkx + t + j * T;
       if (JAC) {
         (h == 0 ? dout : dsp)[o] = cscale(z, out_scale);
