@@ -171,4 +171,4 @@ def test_closure_gains_over_the_coarse_solver():
     res = CL.tendency_gain(c, test_set)
     print("closure held-out", res, flush=True)
     assert np.isfinite(c.loss).all()
-    assert res["gain"] > 0.0, res
+    assert res["gain"] > 0.3, res   # 0.52 measured; a closure that learned nothing scores ~0
