@@ -242,6 +242,27 @@ int nhqg_mode_power(nhqg_ctx *ctx, double *out, int32_t n_out);
  * on the current device; th_bar is copied. */
 int nhqg_upsample(nhqg_ctx *dst, nhqg_ctx *src);
 
+/* The driver's annular barotropic vorticity initial state
+ * (scripts/run_polar.py:115-158) as the context state.  noise: nx*nx
+ * row-major, the caller's standard-normal draw (the driver's
+ * numpy.random.default_rng(seed).standard_normal((Nx, Nx))).  On the device:
+ * rfft2, the Gaussian annulus exp(-((|k| - k_peak)/k_width)^2 / 2) with
+ * k = 0 removed and the 2/3 band, irfft2; when confine_radius > 0 a polar-cap
+ * taper 0.5 (1 - tanh((r - confine_radius) / (4 pi / k_peak))) around the
+ * box centre and the 2/3 band again; scaled to population standard deviation
+ * `amplitude` (over r < confine_radius when confined); its rfft2 is q's T_0
+ * coefficient, everything else zero.  L: the box side.  Uses the context's
+ * FFT scratch; allocates nothing. */
+int nhqg_init_barotropic(nhqg_ctx *ctx, const double *noise, double L, double k_peak,
+                         double k_width, double amplitude, double confine_radius);
+
+/* Gaussian cyclones added to q's T_0 coefficient (scripts/run_polar.py:
+ * 91-112): centers holds count (x0, y0) pairs, the caller's uniform draws on
+ * [0, L); the patch sum_c amplitude exp(-(dx^2 + dy^2) / (2 radius^2)) with
+ * periodic separations is rfft2'd, 2/3-masked and added. */
+int nhqg_inject_cyclones(nhqg_ctx *ctx, const double *centers, int32_t count, double L,
+                         double amplitude, double radius);
+
 /* Component entry points on the device state, for parity testing:
  *   NHQG_OP_EXPLICIT_RHS    aux <- explicit_rhs(state)    solver.py:510-599
  *   NHQG_OP_IMPLICIT_TEND   aux <- implicit_tendency(state) solver.py:615-641

@@ -77,7 +77,7 @@ EXPORTS = (
     "nhqg_mlp_create", "nhqg_mlp_destroy", "nhqg_mlp_n_params", "nhqg_mlp_set_params",
     "nhqg_mlp_get_params", "nhqg_mlp_forward", "nhqg_mlp_gradient", "nhqg_mlp_set_data",
     "nhqg_mlp_train", "nhqg_mlp_last_error", "nhqg_set_closure", "nhqg_mlp_forward_noalloc",
-    "nhqg_mode_power", "nhqg_upsample",
+    "nhqg_mode_power", "nhqg_upsample", "nhqg_init_barotropic", "nhqg_inject_cyclones",
 )
 
 
@@ -109,6 +109,9 @@ def load():
     lib.nhqg_diagnostics.argtypes = [vp, _dp, C.c_int32]
     lib.nhqg_mode_power.argtypes = [vp, _dp, C.c_int32]
     lib.nhqg_upsample.argtypes = [vp, vp]
+    d = C.c_double
+    lib.nhqg_init_barotropic.argtypes = [vp, _dp, d, d, d, d, d]
+    lib.nhqg_inject_cyclones.argtypes = [vp, _dp, C.c_int32, d, d, d]
     lib.nhqg_apply.argtypes = [vp, C.c_int32, C.c_double]
     lib.nhqg_get_aux.argtypes = [vp, _dp, _dp, _dp]
     lib.nhqg_last_step_ms.argtypes = [vp, _dp]
@@ -282,6 +285,25 @@ class Context:
     def upsample_from(self, src: "Context") -> None:
         """This context's state <- src's state spectrally upsampled (device)."""
         self._check(self.lib.nhqg_upsample(self.h, src.h), "nhqg_upsample")
+
+    def init_barotropic(self, noise, L, k_peak, k_width, amplitude, confine_radius=None):
+        """The driver's annular barotropic vorticity state built on the device
+        from the caller's (nx, nx) standard-normal draw."""
+        n = self.grid.Nx
+        noise = np.ascontiguousarray(noise, dtype=np.float64)
+        if noise.shape != (n, n):
+            raise ValueError(f"noise must be ({n}, {n})")
+        self._check(self.lib.nhqg_init_barotropic(
+            self.h, _ptr(noise), float(L), float(k_peak), float(k_width), float(amplitude),
+            0.0 if confine_radius is None else float(confine_radius)), "nhqg_init_barotropic")
+
+    def inject_cyclones(self, centers, L, amplitude, radius):
+        """Gaussian cyclones at the given (x0, y0) centres added to q's T_0
+        coefficient on the device."""
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 2)
+        self._check(self.lib.nhqg_inject_cyclones(
+            self.h, _ptr(c), int(c.shape[0]), float(L), float(amplitude), float(radius)),
+            "nhqg_inject_cyclones")
 
     def mode_power(self):
         """(3, nm): sum_j cc_j |f_j|^2 of the nodal q, w, theta per mode."""

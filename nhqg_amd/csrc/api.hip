@@ -1699,6 +1699,77 @@ int nhqg_upsample(nhqg_ctx* c, nhqg_ctx* src) {
   return NHQG_OK;
 }
 
+// Work arrays of the init entry points in the FFT scratch: the level f
+// (nx*nx doubles), its rfft2 spec (nx*(nx/2+1)) and up to `extra` doubles.
+static int init_scratch(nhqg_ctx* c, size_t extra, double** f, double2** spec, double** tail) {
+  const size_t N = c->Nx, nf = (N * N + 1) / 2 + 32, ns = N * (N / 2 + 1) + 32;
+  if (nf + ns + (extra + 1) / 2 > c->fstride * 17) {
+    c->err = "init: the FFT scratch is too small for this call";
+    return NHQG_ERR_ARG;
+  }
+  *f = (double*)c->X5;
+  *spec = c->X5 + nf;
+  if (tail) *tail = (double*)(c->X5 + nf + ns);
+  return NHQG_OK;
+}
+
+int nhqg_init_barotropic(nhqg_ctx* c, const double* noise, double L, double k_peak,
+                         double k_width, double amplitude, double confine_radius) {
+  if (!c || !noise || !(L > 0.0) || !(k_peak > 0.0) || !(k_width > 0.0) || !(amplitude >= 0.0))
+    return NHQG_ERR_ARG;
+  const int N = c->Nx;
+  double* f;
+  double2* spec;
+  RUN(init_scratch(c, 0, &f, &spec, nullptr));
+  const double k0 = 2.0 * M_PI / L;
+  CK(hipMemcpyAsync(f, noise, sizeof(double) * N * N, hipMemcpyHostToDevice, c->st));
+  // zeta = irfft2(rfft2(noise) * annulus * band)
+  LK(launch_init_fft2(N, true, f, spec, c->o.tw, c->st));
+  LK(launch_init_spec_filter(N, spec, k0, k_peak, k_width, 1, c->st));
+  LK(launch_init_fft2(N, false, f, spec, c->o.tw, c->st));
+  const bool confine = confine_radius > 0.0;
+  if (confine) {
+    // polar-cap taper two initial wavelengths wide, then band-limited again
+    LK(launch_init_taper(N, f, L, confine_radius, 2.0 * 2.0 * M_PI / k_peak, c->st));
+    LK(launch_init_fft2(N, true, f, spec, c->o.tw, c->st));
+    LK(launch_init_spec_filter(N, spec, k0, k_peak, k_width, 0, c->st));
+    LK(launch_init_fft2(N, false, f, spec, c->o.tw, c->st));
+  }
+  LK(launch_init_normalize(N, f, L, confine ? confine_radius : 0.0, amplitude, c->st));
+  LK(launch_init_fft2(N, true, f, spec, c->o.tw, c->st));
+  CK(hipMemsetAsync(c->yq, 0, sizeof(double) * c->Nq * c->ld, c->st));
+  CK(hipMemsetAsync(c->yw, 0, sizeof(double) * c->Ng * c->ld, c->st));
+  CK(hipMemsetAsync(c->yth, 0, sizeof(double) * c->Ng * c->ld, c->st));
+  CK(hipMemsetAsync(c->ythb, 0, sizeof(double) * c->Nq, c->st));
+  LK(launch_init_put_q0(c->nm, c->o.full_idx, spec, (double2*)c->yq, 0, c->st));
+  CK(hipStreamSynchronize(c->st));
+  c->have_state = true;
+  return NHQG_OK;
+}
+
+int nhqg_inject_cyclones(nhqg_ctx* c, const double* centers, int32_t count, double L,
+                         double amplitude, double radius) {
+  if (!c || count < 0 || (count > 0 && !centers) || !(L > 0.0) || !(radius > 0.0))
+    return NHQG_ERR_ARG;
+  if (!c->have_state) {
+    c->err = "nhqg_inject_cyclones: the context has no state";
+    return NHQG_ERR_STATE;
+  }
+  if (count == 0) return NHQG_OK;
+  const int N = c->Nx;
+  double *f, *cen;
+  double2* spec;
+  RUN(init_scratch(c, (size_t)2 * count, &f, &spec, &cen));
+  CK(hipStreamSynchronize(c->st));
+  CK(hipMemcpyAsync(cen, centers, sizeof(double) * 2 * count, hipMemcpyHostToDevice, c->st));
+  LK(launch_init_cyclones(N, f, L, cen, count, amplitude, radius, c->st));
+  LK(launch_init_fft2(N, true, f, spec, c->o.tw, c->st));
+  LK(launch_init_spec_filter(N, spec, 0.0, 0.0, 1.0, 0, c->st));
+  LK(launch_init_put_q0(c->nm, c->o.full_idx, spec, (double2*)c->yq, 1, c->st));
+  CK(hipStreamSynchronize(c->st));
+  return NHQG_OK;
+}
+
 int nhqg_mode_power(nhqg_ctx* c, double* out, int32_t n_out) {
   if (!c || !out) return NHQG_ERR_ARG;
   if (!c->have_state) return NHQG_ERR_STATE;

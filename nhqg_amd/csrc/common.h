@@ -155,6 +155,20 @@ struct FluxArgs {
 };
 // pass: 0 all three passes, 1 col_inverse, 2 row products, 3 col_forward
 hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s);
+// single-level 2-D real transforms and init steps (init.hip): forward
+// spec = rfft2(f), else f = irfft2(spec) with spec overwritten; spec is
+// N x (N/2+1) in rfft2 layout
+hipError_t launch_init_fft2(int N, bool forward, double* f, double2* spec, const double2* tw,
+                            hipStream_t s);
+hipError_t launch_init_spec_filter(int N, double2* spec, double k0, double k_peak,
+                                   double k_width, int annulus, hipStream_t s);
+hipError_t launch_init_taper(int N, double* f, double L, double R, double width, hipStream_t s);
+hipError_t launch_init_normalize(int N, double* f, double L, double R, double amplitude,
+                                 hipStream_t s);
+hipError_t launch_init_cyclones(int N, double* f, double L, const double* centers, int count,
+                                double amplitude, double radius, hipStream_t s);
+hipError_t launch_init_put_q0(int nm, const int* full_idx, const double2* spec, double2* q0,
+                              int add, hipStream_t s);
 // the 3/2 rule's passes (a.N = 3P, P = 8..512; fft_pad.hip)
 hipError_t launch_flux_pad(const FluxArgs& a, int pass, hipStream_t s);
 // rowmax: Nq*N row maxima of the speed from psi (a.qn); with a.wn and a.thn

@@ -202,6 +202,31 @@ class Stepper:
         self._band().upsample_from(other._current())
         self._pending_full = False
 
+    def init_barotropic(self, seed: int, k_peak: float, k_width: float, amplitude: float,
+                        confine_radius: float | None = None) -> None:
+        """The driver's annular barotropic vorticity initial state
+        (scripts/run_polar.py:115-158) built on the device: the white noise
+        is numpy's default_rng(seed).standard_normal((Nx, Nx)), as the driver
+        draws it; filtering, taper, normalisation and placement run in
+        nhqg_init_barotropic."""
+        noise = np.random.default_rng(seed).standard_normal((self.grid.Nx, self.grid.Nx))
+        self._band().init_barotropic(noise, self.grid.L, k_peak, k_width, amplitude,
+                                     confine_radius)
+        self._pending_full = False
+
+    def inject_cyclones(self, amplitude: float, radius: float, count: int = 1,
+                        rng: np.random.Generator | None = None) -> None:
+        """Gaussian cyclones added to the resident state's barotropic PV at
+        positions drawn as the driver draws them (rng.uniform(0, L, 2) per
+        cyclone, scripts/run_polar.py:91-112), on the device."""
+        if self.ctx is None:
+            raise RuntimeError("Stepper.inject_cyclones before a state is set")
+        rng = np.random.default_rng() if rng is None else rng
+        L = self.grid.L
+        centers = np.array([rng.uniform(0.0, L, size=2) for _ in range(int(count))])
+        if centers.size:
+            self._current().inject_cyclones(centers, L, amplitude, radius)
+
     def close(self) -> None:
         for c in (self.ctx, self._full):
             if c is not None:
