@@ -167,6 +167,12 @@ typedef struct {
   const int32_t *kidx_of_x_pad, *rows_pad;
   const double *twiddle_pad;
   const double *eta_phys_pad, *eta_x_phys_pad, *eta_y_phys_pad, *sponge_phys_pad;
+  /* vertical_cutoff_n (nhqg/solver.py:177-191), Ng x Ng each, or NULL for
+   * none: cutoff_w = w_pinv proj_w diag(n <= cutoff) w_stencil and cutoff_th
+   * = d_pinv proj_d diag(n <= cutoff) d_stencil, applied to w and theta at
+   * the end of every step (_finalize_state) and to every RK4 stage state
+   * (_apply_bcs). */
+  const double *cutoff_w, *cutoff_th;
 } nhqg_operators;
 
 /* Diagnostics vector layout returned by nhqg_diagnostics (length

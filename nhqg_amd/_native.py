@@ -61,6 +61,7 @@ _OPS_FIELDS = [
     ("pweight_pad", _dp), ("partner_pad", _ip),
     ("kidx_of_x_pad", _ip), ("rows_pad", _ip), ("twiddle_pad", _dp), ("eta_phys_pad", _dp),
     ("eta_x_phys_pad", _dp), ("eta_y_phys_pad", _dp), ("sponge_phys_pad", _dp),
+    ("cutoff_w", _dp), ("cutoff_th", _dp),
 ]
 
 

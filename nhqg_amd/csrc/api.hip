@@ -26,6 +26,8 @@ struct Ops {
   GemmOp V, VSw, VSd, Vinv, Pw, Pd, Qw, GSw, W2S, T2S, S2T, Psbp, Mtw, Mwt, Mth2w, Sw, Sd, row_cc;
   // Dirichlet-top shell solves in the eigenbasis of B0 (eigQ.At set)
   GemmOp eigQ, eigQinv;
+  // vertical_cutoff_n: w and theta maps of _apply_vertical_cutoff (Cw.At set)
+  GemmOp Cw, Cth;
   double *lam, *mal, *mc, *dtab;
   double *thb2s, *s2thb, *proj_d;
   double *d1b, *lb, *cc, *cc_sbp;
@@ -671,6 +673,20 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
   return NHQG_OK;
 }
 
+// _apply_vertical_cutoff (nhqg/solver.py:177-191): w <- Cw w and theta <-
+// Cth theta, Cw = w_pinv proj_w diag(n <= cutoff) w_stencil (Cth likewise
+// with the Dirichlet maps); the identity without a cutoff
+int vertical_cutoff(nhqg_ctx* c, double* w, double* th) {
+  const Ops& o = c->o;
+  if (!o.Cw.At) return NHQG_OK;
+  RUN(gemm_jobs(c, {Job{&o.Cw, w, c->tmpg, epi_make(1.0)},
+                    Job{&o.Cth, th, c->rhs_w, epi_make(1.0)}}));
+  const size_t bytes = sizeof(double) * c->Ng * c->ld;
+  CK(hipMemcpyAsync(w, c->tmpg, bytes, hipMemcpyDeviceToDevice, c->st));
+  CK(hipMemcpyAsync(th, c->rhs_w, bytes, hipMemcpyDeviceToDevice, c->st));
+  return NHQG_OK;
+}
+
 int sanitize(nhqg_ctx* c) {
   const Ops& o = c->o;
   PL(CL_SMALL, 0.0, launch_sanitize(c->Nq, c->ld, c->P.n_pairs, o.pairs, 0, c->yq, c->st));
@@ -813,6 +829,9 @@ int enqueue_step_body(nhqg_ctx* c) {
   PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, legacy ? o.msolve : o.proj_d, c->Rthb, c->ythb,
                                         c->st));
   if (em) RUN(corrector(c, c->yw, c->yth, c->ythb, c->yth, c->ythb, al, fk));
+  // _finalize_state (solver.py:993-999): zero_mean (in sanitize, which
+  // commutes with the per-mode cutoff) and the optional vertical cutoff
+  RUN(vertical_cutoff(c, c->yw, c->yth));
   RUN(sanitize(c));
   return NHQG_OK;
 }
@@ -921,6 +940,8 @@ int rk4_step(nhqg_ctx* c) {
       PL(CL_SMALL, 0.0, launch_small_lincomb(Nq, c->Rthb, 1.0, c->ythb, a, c->ktb, 0.0, nullptr,
                                              0.0, nullptr, c->st));
       PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->stb, c->st));
+      // _apply_bcs ends with the vertical cutoff (solver.py:1261-1273)
+      RUN(vertical_cutoff(c, out[1], out[2]));
     }
     return NHQG_OK;
   };
@@ -957,6 +978,7 @@ int rk4_step(nhqg_ctx* c) {
   PL(CL_SMALL, 0.0, launch_small_lincomb(Nq, c->Rthb, 1.0, c->acctb, dt / 6.0, c->ktb, 0.0,
                                          nullptr, 0.0, nullptr, c->st));
   PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->Rthb, c->ythb, c->st));
+  RUN(vertical_cutoff(c, c->yw, c->yth));
   RUN(sanitize(c));
   return NHQG_OK;
 }
@@ -1102,6 +1124,16 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(upload(c, &o.cc_sbp, h->cc_sbp, (size_t)Nq));
   RUN(upload_op(c, &o.Sw, h->Sw, Nq, Ng, mt));
   RUN(upload_op(c, &o.Sd, h->Sd, Nq, Ng, mt));
+  if ((h->cutoff_w == nullptr) != (h->cutoff_th == nullptr)) {
+    c->err = "the vertical cutoff needs both cutoff_w and cutoff_th";
+    return NHQG_ERR_ARG;
+  }
+  if (h->cutoff_w) {
+    RUN(upload_op(c, &o.Cw, h->cutoff_w, Ng, Ng, mt));
+    RUN(upload_op(c, &o.Cth, h->cutoff_th, Ng, Ng, mt));
+    RUN(upload_op_split(c, &o.Cw, h->cutoff_w, false));
+    RUN(upload_op_split(c, &o.Cth, h->cutoff_th, false));
+  }
   if (p->w_neumann) {
     if (!h->Mtw || !h->Mwt || !h->Mth2w) {
       c->err = "w_neumann requires Mtw, Mwt and Mth2w";

@@ -14,7 +14,8 @@ on the zero-padded 3Nx/2 grid), ``vertical_dealiasing='none'``,
 ``'jacobian'``, ``imex_scheme='ars222'``, ``sbp_transfer_mode='interp'``,
 ``thermal_closure`` either ``'fixed_conduction'`` or ``'evolve_mean'``,
 ``mean_exchange_discretization`` either ``'balanced_sbp2_pc'`` or ``'legacy'``,
-either ``w_bc_top``, power-of-two ``Nx`` in [16, 1024], ``Nz`` <= 256.
+either ``w_bc_top``, any ``vertical_cutoff_n``, power-of-two ``Nx`` in [16, 1024],
+``Nz`` <= 256.
 """
 
 from __future__ import annotations
@@ -416,8 +417,6 @@ def _check_config(cfg: NHQGConfig) -> None:
         raise NotImplementedError(
             "the MI355X step is built for mean_exchange_discretization='balanced_sbp2_pc' or "
             f"'legacy' (got {cfg.mean_exchange_discretization!r})")
-    if cfg.vertical_cutoff_n is not None and cfg.vertical_cutoff_n < cfg.Nz:
-        raise NotImplementedError("vertical_cutoff_n is not built")
     if cfg.Nx < 16 or cfg.Nx > 1024 or cfg.Nx & (cfg.Nx - 1):
         raise NotImplementedError("Nx must be a power of two in [16, 1024]")
     if cfg.Nz < 3 or cfg.Nz > 256:
@@ -751,6 +750,13 @@ def device_operators(g: Grid, lay: ModeLayout) -> dict:
                mean_flux=-g.mean_temp_eps_sq * (GZ @ Vi))
     k = np.arange(Nx)
     ops["twiddle"] = np.stack([np.cos(2.0 * np.pi * k / Nx), -np.sin(2.0 * np.pi * k / Nx)], 1)
+    # vertical_cutoff_n (nhqg/solver.py:177-191): keep Chebyshev degrees
+    # n <= cutoff of w and theta, re-project onto the boundary conditions
+    cut = g.vertical_cutoff_n
+    if cut is not None and cut < g.Nz:
+        keep = (np.arange(n) <= cut).astype(np.float64)[:, None]
+        ops["cutoff_w"] = g.w_pinv @ g.proj_w @ (keep * Sw)
+        ops["cutoff_th"] = g.dirichlet_pinv @ g.proj_dirichlet @ (keep * Sd)
     # shell factors of the touched shells: built and inverted on the device
     # in FP64 from B0, K and three scalars per shell (default), or inverted
     # here with LAPACK and uploaded transposed

@@ -72,8 +72,6 @@ def unsupported_reason(ref) -> str | None:
             return f"{name}={getattr(ref, name)!r}"
     if ref.mean_exchange_discretization not in ("legacy", "balanced_sbp2_pc"):
         return f"mean_exchange_discretization={ref.mean_exchange_discretization!r}"
-    if ref.vertical_cutoff_n is not None and ref.vertical_cutoff_n < ref.Nz:
-        return "vertical_cutoff_n"
     if ref.Nx < 16 or ref.Nx > 1024 or ref.Nx & (ref.Nx - 1):
         return f"Nx={ref.Nx}"
     if ref.Npad != 3 * ref.Nx // 2:

@@ -29,6 +29,7 @@ CONFIGS = {
     # the reference's default dealiasing: products on the 3Nx/2 grid
     "rule32_trap_sponge": lambda: cases.trap_sponge_cfg(horizontal_dealiasing="32_rule",
                                                         nonlinear_advection="jacobian"),
+    "cutoff": lambda: cases.fixture_cfg(vertical_cutoff_n=10),
 }
 
 
@@ -110,7 +111,8 @@ def test_binding_step_is_bit_identical_to_the_package_step(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["fixture", "trap_sponge", "neumann", "rule32_trap_sponge"])
+@pytest.mark.parametrize("name", ["fixture", "trap_sponge", "neumann", "rule32_trap_sponge",
+                                  "cutoff"])
 def test_binding_on_oracle_operators_matches_oracle(name):
     cfg = CONFIGS[name]()
     print("binding vs oracle", name, flush=True)

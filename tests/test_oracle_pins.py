@@ -697,3 +697,47 @@ def test_sbp_heat_flux_profile_converges_at_second_order():
         err[Nz] = abs(O.diagnostics(s, o)["vol_avg_tw_dealiased"] - 0.25)
     assert 3.0 < err[16] / err[32] < 5.0 and 3.0 < err[32] / err[64] < 5.0
     assert err[64] < 4e-4
+
+
+# ---------------------------------------------------------------------------
+# vertical_cutoff_n (solver.py:177-191)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("top", ["dirichlet", "neumann"])
+def test_vertical_cutoff_is_a_boundary_preserving_projection(top):
+    """The cutoff keeps Chebyshev degrees n <= cutoff and re-imposes the
+    boundary conditions: it is idempotent, the identity on fields with no
+    content above the cutoff (which an off-by-one in the degree test
+    breaks), and its result satisfies the boundary conditions of w and
+    theta."""
+    N, cut = 16, 9
+    o = O.build_operators(_cfg(Nx=16, Nz=N, L=20.0, vertical_cutoff_n=cut, w_bc_top=top,
+                               horizontal_dealiasing="23_rule"), with_shells=False)
+    rng = np.random.default_rng(4)
+    shape = (N - 1, 16, 9)
+    w = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    th = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    s = O.OState(np.zeros((N + 1, 16, 9), complex), w, th, np.zeros(N + 1))
+    a = O.vertical_cutoff(s, o)
+    b = O.vertical_cutoff(a, o)
+    assert np.abs(b.w - a.w).max() <= 1e-13 * np.abs(a.w).max()
+    assert np.abs(b.th - a.th).max() <= 1e-13 * np.abs(a.th).max()
+    assert np.abs(a.w - w).max() > 1e-2          # it did remove something
+    # Galerkin index j spans degrees j and j+2: indices <= cut-2 stay below the cutoff
+    low = np.zeros(shape, complex)
+    low[:cut - 1] = w[:cut - 1]
+    c = O.vertical_cutoff(O.OState(s.q, low, low, s.thb), o)
+    assert np.abs(c.w - low).max() <= 1e-13 * np.abs(low).max()
+    assert np.abs(c.th - low).max() <= 1e-13 * np.abs(low).max()
+    # boundary conditions of the result on the CGL grid (Z = 1 first, Z = 0 last)
+    wn = O.vmat(o.V, O.vmat(o.w_stencil, a.w))
+    tn = O.vmat(o.V, O.vmat(o.S_d, a.th))
+    sc = np.abs(wn).max()
+    assert np.abs(wn[-1]).max() <= 1e-12 * sc                     # w = 0 at the bottom
+    assert np.abs(tn[0]).max() <= 1e-12 * np.abs(tn).max()        # theta = 0 at both ends
+    assert np.abs(tn[-1]).max() <= 1e-12 * np.abs(tn).max()
+    if top == "dirichlet":
+        assert np.abs(wn[0]).max() <= 1e-12 * sc
+    else:
+        dw = O.vmat(o.V, O.vmat(o.G_Z, O.vmat(o.w_stencil, a.w)))
+        assert np.abs(dw[0]).max() <= 1e-11 * np.abs(dw).max()   # dw/dZ = 0 at the top
