@@ -36,6 +36,7 @@ struct Ops {
   double *kx_row, *ky_col, *kx_mode, *ksq, *inv_denom, *iaq, *iath, *idq, *diss_q, *diss_w,
       *diss_th, *mask, *pw, *ones, *pw_ke, *pw_cc;
   double *shell_inv, *eta_phys, *sponge_phys, *eta_x, *eta_y;
+  double2* eta_rows;   // the trap in the Nx = 256 row pass's input form
   double2* tw;
   // mean temperature (evolve_mean): V G_Z, tau solve, diffusion, flux map
   double *vgz, *msolve, *mdiff, *mflux;
@@ -436,6 +437,7 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   a.inv_denom = o.inv_denom;
   a.mask = o.mask;
   a.eta_phys = o.eta_phys;
+  a.eta_rows = o.eta_rows;
   a.sponge_phys = o.sponge_phys;
   a.jac = c->P.advection;
   a.eta_x = o.eta_x;
@@ -1296,6 +1298,11 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
     o.fkidx = o.kidx_of_x;
     o.frows = o.rows;
     o.ftw = o.tw;
+  }
+  o.eta_rows = nullptr;
+  if (!pad && c->Nx == 256 && o.eta_phys && p->advection == 0) {
+    RUN(dalloc(c, &o.eta_rows, (size_t)c->Nx * c->nky));
+    CK(launch_eta_rows(c->Nx, c->nky, o.eta_phys, o.tw, o.eta_rows, c->st));
   }
   // mean temperature: the legacy exchange needs all four maps, RK4 under
   // evolve_mean the diffusion

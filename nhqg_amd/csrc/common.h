@@ -152,6 +152,9 @@ struct FluxArgs {
   const double2* tw;
   double2 *X5, *Y6;
   size_t fstride;
+  // the trap's rows in X5's form (launch_eta_rows), or null: the Nx = 256
+  // flux-form row pass adds eta to q' in spectral form when it is given
+  const double2* eta_rows = nullptr;
 };
 // pass: 0 all three passes, 1 col_inverse, 2 row products, 3 col_forward
 hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s);
@@ -169,6 +172,12 @@ hipError_t launch_init_cyclones(int N, double* f, double L, const double* center
                                 double amplitude, double radius, hipStream_t s);
 hipError_t launch_init_put_q0(int nm, const int* full_idx, const double2* spec, double2* q0,
                               int add, hipStream_t s);
+// the flux-form row pass at N = 256 with 16 threads per transform
+// (fft_r16.hip); false when a does not fit it
+bool launch_row_flux16(const FluxArgs& a, hipStream_t s);
+// out[x][ky] = N * sum_y eta_phys[x][y] exp(-2 pi i ky y / N), ky < nky
+hipError_t launch_eta_rows(int N, int nky, const double* eta_phys, const double2* twg,
+                           double2* out, hipStream_t s);
 // the 3/2 rule's passes (a.N = 3P, P = 8..512; fft_pad.hip)
 hipError_t launch_flux_pad(const FluxArgs& a, int pass, hipStream_t s);
 // rowmax: Nq*N row maxima of the speed from psi (a.qn); with a.wn and a.thn

@@ -32,6 +32,10 @@
 // products in the tiny high Chebyshev coefficients, which d/dZ amplifies.
 #include "fft_team.h"
 
+#ifndef NHQG_ROW16
+#define NHQG_ROW16 1   // the 16-thread row pass of fft_r16.hip at N = 256
+#endif
+
 // Threads per column-pass workgroup: 8 columns (128-byte runs of every
 // X5 / Y6 row) at N = 256 and 512, 4 at N = 1024, and 256 threads (1024/N
 // columns) below.  At N = 256, 4-column blocks wrote 21% more than the
@@ -712,7 +716,7 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
     else
       hipLaunchKernelGGL((row_jac<N, SP, false>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_x, a.eta_y, a.sponge_phys, a.tw);
-  } else if (pass == 0 || pass == 2) {
+  } else if ((pass == 0 || pass == 2) && !(NHQG_ROW16 && N == 256 && launch_row_flux16(a, s))) {
     dim3 g2((N / 2 + TEAMS - 1) / TEAMS, a.nq);   // two rows per team
     if (a.eta_phys)
       hipLaunchKernelGGL((row_flux2<N, SP, true>), g2, dim3(256), 0, s, a.nky, a.ng, a.X5,

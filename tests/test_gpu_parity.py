@@ -137,6 +137,11 @@ def test_one_step(name):
 
 
 WIDE = {
+    # Nx = 256: the 16-thread row pass, with and without the trap and sponge
+    "trap_sponge_256": lambda: cases.trap_sponge_cfg(Nx=256, Nz=10),
+    "trap_256": lambda: cases.trap_sponge_cfg(Nx=256, Nz=8, sponge_rate=0.0),
+    "sponge_256": lambda: cases.trap_sponge_cfg(Nx=256, Nz=8, gamma=0.0, sponge_r_start=7.0),
+    "fixture_256": lambda: cases.fixture_cfg(Nx=256, Nz=8),
     "trap_sponge_512": lambda: cases.trap_sponge_cfg(Nx=512, Nz=10),
     "fixture_1024": lambda: cases.fixture_cfg(Nx=1024, Nz=8),
 }
@@ -144,8 +149,8 @@ WIDE = {
 
 @pytest.mark.parametrize("name", list(WIDE))
 def test_one_step_wide(name):
-    """The Nx = 512 and 1024 instantiations of the FFT passes, at small Nz
-    so the oracle step stays at seconds."""
+    """The Nx = 256, 512 and 1024 instantiations of the FFT passes, at small
+    Nz so the oracle step stays at seconds."""
     cfg = WIDE[name]()
     g = make_grid(cfg)
     o = O.build_operators(cfg)
