@@ -66,6 +66,8 @@ def profiled_traffic(cls: str, path: str = TRAFFIC_FILE):
 def profile_steps(ctx, n: int = 3) -> dict:
     """Per-class (ms, launches, work) of n eagerly profiled steps, the median
     ms of each class: one profiled step varies by ~5% from run to run."""
+    import numpy as np
+
     runs = [ctx.profile_step() for _ in range(n)]
     return {k: (float(np.median([r[k][0] for r in runs])), v[1], v[2])
             for k, v in runs[0].items()}
