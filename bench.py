@@ -209,10 +209,14 @@ def cpu_c1(cores: int, steps: int = 200) -> dict:
     return out
 
 
-def mixed_leg(cfg, st0, args, ref, dref) -> dict:
+def mixed_leg(cfg, st0, args, ref, dref, plain_ms: float) -> dict:
     """The mixed-precision step (int8 slice contractions) on the same state,
     warmup and timed steps as the plain run; errors against the plain run's
-    state after the same number of steps."""
+    state after the same number of steps.  speedup_ceiling: the plain step's
+    time over the mixed step's with the profiled time of its int8
+    contractions removed (free slice products), and with every contraction
+    removed (the FP64 SBP transfers too): the FFT products, the SBP 1-D part,
+    the Parseval means and the small kernels stay FP64 in this form."""
     import numpy as np
 
     from nhqg_amd import Stepper, make_grid
@@ -244,6 +248,16 @@ def mixed_leg(cfg, st0, args, ref, dref) -> dict:
            "rel_error_vs_plain_fp64": err,
            "kernels": {k: {"ms_per_step": v[0], "launches": v[1]} for k, v in prof.items()},
            "roofline": rl.get("gemm_i8_mixed")}
+    mixed_ms = out["ms_per_step"]
+    i8_ms = prof.get("gemm_i8_mixed", (0.0, 0, 0.0))[0]
+    f64_ms = prof.get("gemm_f64_mfma", (0.0, 0, 0.0))[0]
+    out["speedup_vs_plain"] = plain_ms / mixed_ms
+    out["speedup_ceiling"] = {
+        "int8_contractions_free": plain_ms / max(mixed_ms - i8_ms, 1e-9),
+        "all_contractions_free": plain_ms / max(mixed_ms - i8_ms - f64_ms, 1e-9),
+        "note": "plain ms/step over the mixed ms/step less the eagerly profiled ms/step of the "
+                "int8 contractions (and of the FP64 SBP contractions): what the step would gain "
+                "from infinitely fast products, the FP64 FFT products and SBP chain unchanged"}
     g.close()
     return out
 
@@ -553,7 +567,7 @@ def main() -> int:
                             "WRITE_SIZE summary (" + os.path.relpath(TRAFFIC_FILE, ROOT) + "), "
                             "calibrated on a 1 GiB copy")
     if args.mixed_slices:
-        res["mixed"] = mixed_leg(cfg, st0, args, final, diag)
+        res["mixed"] = mixed_leg(cfg, st0, args, final, diag, res["ms_per_step"])
     if headline and not args.no_closure:
         res["nz64_closure"] = closure_leg(cfg, st0, args, (final, diag), g)
     if headline and not args.no_published:
