@@ -156,24 +156,34 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 @dataclass
 class Closure:
+    """A trained network and its scales.  ``gain`` multiplies the tendency
+    the network adds in the step: relaxation pairs fit the mean correction
+    over their horizon, i.e. a relaxation time of the horizon, and the gain
+    shortens it (chosen on a validation run by ``select_gain``; 1 applies the
+    fitted correction as it is)."""
     mlp: "_native.MLP"
     n0: int
     in_scale: float
     out_scale: float
     loss: np.ndarray
+    gain: float = 1.0
 
     def predict(self, x: np.ndarray) -> np.ndarray:
-        """Tendency for raw inputs (FP32 weights)."""
-        return self.out_scale * self.mlp.forward(self.in_scale * np.asarray(x)).astype(np.float64)
+        """Tendency for raw inputs (FP32 weights), in batches the device
+        network accepts."""
+        x = self.in_scale * np.asarray(x)
+        n = self.mlp.max_batch
+        y = [self.mlp.forward(x[i:i + n]) for i in range(0, x.shape[0], n)]
+        return self.out_scale * np.concatenate(y).astype(np.float64)
 
     def attach(self, grid: Grid) -> None:
-        grid.attach_closure(self.mlp, self.n0, self.in_scale, self.out_scale)
+        grid.attach_closure(self.mlp, self.n0, self.in_scale, self.out_scale * self.gain)
 
     def save(self, path: str, **meta) -> None:
         """FP32 parameters, scales and the top-level offset as .npz (loaded
         without pickle)."""
         np.savez(path, params=self.mlp.get_params(), n0=self.n0, in_scale=self.in_scale,
-                 out_scale=self.out_scale, loss=np.asarray(self.loss, np.float32),
+                 out_scale=self.out_scale, gain=self.gain, loss=np.asarray(self.loss, np.float32),
                  **{k: np.asarray(v) for k, v in meta.items()})
 
     @staticmethod
@@ -184,8 +194,9 @@ class Closure:
             mlp = _native.MLP(N_TOP, WIDTH, N_TOP, max_batch=4096, odd=True)
             mlp.set_params(d["params"])
             n0 = int(d["n0"]) if nz is None else nz + 1 - N_TOP
+            gain = float(d["gain"]) if "gain" in d.files else 1.0
             return Closure(mlp, n0, float(d["in_scale"]), float(d["out_scale"]),
-                           np.array(d["loss"]))
+                           np.array(d["loss"]), gain)
 
 
 def train(pairs: PairSet, steps: int = 3000, batch: int = 512, lr: float = 1e-3,
@@ -315,9 +326,49 @@ def judge(cfg_f, cfg_c, state0: State, n_steps: int, every: int, closure: Closur
     return out
 
 
+def select_gain(cfg_f, cfg_c, state0: State, c: Closure, gains, n_steps: int, every: int
+                ) -> dict:
+    """Validation run for the closure's gain: one plain fine run from state0
+    and one coarse run per candidate gain against it.  Scores each gain by
+    the larger of its mean and final top-16 psi-KE error, the two figures
+    the main run reports, and returns the scores with the best gain."""
+    from nhqg_amd.grid import make_grid
+
+    gf = make_grid(cfg_f, shells="band")
+    fine = Stepper(gf)
+    fine.set_state(state0)
+    sf0 = fine.state()
+    runs = []
+    for b in gains:
+        g = make_grid(cfg_c, shells="band")
+        sp = Stepper(g)
+        sp.set_state(filter_state(sf0, gf, g))
+        g.attach_closure(c.mlp, c.n0, c.in_scale, c.out_scale * float(b))
+        runs.append((float(b), g, sp, []))
+    done = 0
+    while done < n_steps:
+        n = min(every, n_steps - done)
+        fine.run(n)
+        for _, _, sp, _ in runs:
+            sp.run(n)
+        done += n
+        e_f = psi_ke_spectrum(fine.state().q_hat, gf)[:cfg_c.Nz + 1]
+        for _, g, sp, errs in runs:
+            errs.append(_spectrum_errors(psi_ke_spectrum(sp.state().q_hat, g), e_f)["psi_ke_top16"])
+    scores = {b: {"top16_mean": float(np.mean(e)), "top16_final": float(e[-1]),
+                  "score": max(float(np.mean(e)), float(e[-1]))} for b, _, _, e in runs}
+    for _, g, sp, _ in runs:
+        sp.close()
+        g.close()
+    fine.close()
+    gf.close()
+    best = min(scores, key=lambda b: scores[b]["score"])
+    return {"gain": best, "scores": scores}
+
+
 def main(argv=None) -> int:
-    """Train on pairs from one seeded fine run, judge on a run from another
-    seed; prints one JSON object."""
+    """Train on pairs from one seeded fine run, choose the gain on a run from
+    a second seed, judge on a run from a third; prints one JSON object."""
     import argparse
     import json
     import time
@@ -341,6 +392,9 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed-train", type=int, default=0)
     ap.add_argument("--seed-judge", type=int, default=10)
+    ap.add_argument("--seed-val", type=int, default=20, help="seed of the gain-selection run")
+    ap.add_argument("--gains", type=str, default="1,2,3,4,5,6,8,10,12",
+                    help="candidate gains (comma separated); a single value skips the selection")
     ap.add_argument("--save", type=str, default=None, help="write the trained closure (.npz)")
     ap.add_argument("--horizon", type=int, default=400,
                     help="1: one-step pairs; n > 1: n-step relaxation pairs from coarse runs")
@@ -367,6 +421,19 @@ def main(argv=None) -> int:
     t_train = time.perf_counter() - t0
     print(f"trained in {t_train:.1f} s, loss {c.loss[:20].mean():.3e} -> {c.loss[-100:].mean():.3e}",
           flush=True)
+    gains = [float(x) for x in args.gains.split(",")]
+    t0 = time.perf_counter()
+    if len(gains) > 1:
+        gv = make_grid(cfg_f, shells="band")
+        s_val = seeded(gv, args.seed_val)
+        gv.close()
+        sel = select_gain(cfg_f, cfg_c, s_val, c, gains, args.steps, args.every)
+    else:
+        sel = {"gain": gains[0], "scores": {}}
+    c.gain = sel["gain"]
+    t_select = time.perf_counter() - t0
+    print(f"gain {c.gain} selected in {t_select:.1f} s: "
+          + json.dumps({str(k): v for k, v in sel["scores"].items()}), flush=True)
     if args.save:
         c.save(args.save, nx=args.nx, nz_fine=args.nz_fine, nz_coarse=args.nz_coarse,
                horizon=args.horizon, pairs=pairs.x.shape[0])
@@ -377,7 +444,10 @@ def main(argv=None) -> int:
     res = judge(cfg_f, cfg_c, s_judge, args.steps, args.every, closure=c)
     t_judge = time.perf_counter() - t0
     res["config"] = vars(args)
-    res["timing_s"] = {"pairs": t_pairs, "train": t_train, "judge": t_judge}
+    res["timing_s"] = {"pairs": t_pairs, "train": t_train, "select_gain": t_select,
+                       "judge": t_judge}
+    res["gain_selection"] = {"gain": c.gain, "seed": args.seed_val,
+                             "scores": {str(k): v for k, v in sel["scores"].items()}}
     res["train_loss"] = {"first20": float(c.loss[:20].mean()),
                          "last100": float(c.loss[-100:].mean())}
     print(json.dumps(res["summary"], indent=1), flush=True)

@@ -489,7 +489,12 @@ int nhqg_mlp_get_params(nhqg_mlp* m, float* w) {
 }
 
 int nhqg_mlp_forward(nhqg_mlp* m, const float* x, int32_t batch, float* y) {
-  if (!m || !x || !y || batch < 1 || batch > m->max_batch) return NHQG_ERR_ARG;
+  if (!m) return NHQG_ERR_ARG;
+  if (!x || !y || batch < 1 || batch > m->max_batch) {
+    m->err = "nhqg_mlp_forward: null pointer, or batch outside [1, " +
+             std::to_string(m->max_batch) + "]";
+    return NHQG_ERR_ARG;
+  }
   MK(hipMemcpyAsync(m->x, x, (size_t)batch * m->n_in * sizeof(float), hipMemcpyHostToDevice,
                     m->st));
   int r = forward(m, batch);
@@ -502,7 +507,12 @@ int nhqg_mlp_forward(nhqg_mlp* m, const float* x, int32_t batch, float* y) {
 
 int nhqg_mlp_gradient(nhqg_mlp* m, const float* x, const float* t, int32_t batch, float* grad,
                       float* loss) {
-  if (!m || !x || !t || !grad || !loss || batch < 1 || batch > m->max_batch) return NHQG_ERR_ARG;
+  if (!m) return NHQG_ERR_ARG;
+  if (!x || !t || !grad || !loss || batch < 1 || batch > m->max_batch) {
+    m->err = "nhqg_mlp_gradient: null pointer, or batch outside [1, " +
+             std::to_string(m->max_batch) + "]";
+    return NHQG_ERR_ARG;
+  }
   MK(hipMemcpyAsync(m->x, x, (size_t)batch * m->n_in * sizeof(float), hipMemcpyHostToDevice,
                     m->st));
   MK(hipMemcpyAsync(m->t, t, (size_t)batch * m->n_out * sizeof(float), hipMemcpyHostToDevice,

@@ -150,6 +150,9 @@ def test_c4_coarse_64_with_closure_against_256_cubed():
     res = judge(cfg_f, cfg_c, s, 1000, 500, closure=c)["summary"]
     say("  ", res)
     plain, clos = res["coarse"], res["coarse+closure"]
-    assert clos["psi_ke_top16_err_mean"] < plain["psi_ke_top16_err_mean"]
+    # the committed weights carry their validated gain (4): over these 1000
+    # steps the top-16 error falls to about a third of the plain run's
+    assert c.gain > 1.0
+    assert clos["psi_ke_top16_err_mean"] < 0.6 * plain["psi_ke_top16_err_mean"]
     assert clos["psi_ke_spectrum_err_final"] < 0.05
     assert clos["Nu_d_rel_err_final"] < 0.05
