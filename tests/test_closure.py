@@ -172,3 +172,52 @@ def test_closure_gains_over_the_coarse_solver():
     print("closure held-out", res, flush=True)
     assert np.isfinite(c.loss).all()
     assert res["gain"] > 0.3, res   # 0.52 measured; a closure that learned nothing scores ~0
+
+
+@pytest.mark.gpu
+def test_closure_gain_scales_the_step_term_and_survives_save_load(tmp_path):
+    """Closure.gain multiplies the tendency the attached network adds to E_q
+    (a gain of 2 doubles the increment, a power of two: exactly up to the
+    rounding of the sums), is saved with the weights, and loads as 1 from a
+    file without it; predict stays the fitted, unscaled map in batches
+    beyond the device network's max_batch."""
+    from nhqg_amd import _native
+    m = _native.MLP(16, 128, 16, max_batch=4096, odd=True)
+    m.set_params(_perturbed_params(5))
+    c = CL.Closure(m, 0, 3.0e3, 2.0e-4, np.zeros(1), gain=1.0)
+    cfg = cases.fixture_cfg(Nz=24)
+    g = make_grid(cfg)
+    c.n0 = g.Nz + 1 - 16
+    s = cases.active_state(cfg)
+    ctx = g.context(full=False)
+    ctx.set_state(s.q, s.w, s.th, s.thb)
+    ctx.apply(_native.OP_EXPLICIT_RHS)
+    e0 = ctx.get_aux()[0]
+    inc = {}
+    for gain in (1.0, 2.0):
+        c.gain = gain
+        c.attach(g)
+        ctx.apply(_native.OP_EXPLICIT_RHS)
+        inc[gain] = ctx.get_aux()[0] - e0
+    g.attach_closure(None, 0, 1.0, 1.0)
+    big = np.abs(inc[2.0]).max()
+    assert big > 0.0
+    # the sums E + inc round at the scale of the increments (half an ulp
+    # of them measured)
+    assert np.abs(inc[2.0] - 2.0 * inc[1.0]).max() <= 1e-15 * big
+    # save / load keeps the gain; a file written without one loads gain 1
+    c.gain = 4.0
+    path = str(tmp_path / "c.npz")
+    c.save(path, note="test")
+    d = CL.Closure.load(path)
+    assert d.gain == 4.0 and d.n0 == c.n0 and d.in_scale == c.in_scale
+    assert np.array_equal(d.mlp.get_params(), c.mlp.get_params())
+    with np.load(path) as z:
+        np.savez(str(tmp_path / "old.npz"), **{k: z[k] for k in z.files if k != "gain"})
+    assert CL.Closure.load(str(tmp_path / "old.npz")).gain == 1.0
+    x = np.random.default_rng(2).standard_normal((9000, 16)) * 1e-3
+    y = c.predict(x)
+    assert y.shape == (9000, 16)
+    assert np.array_equal(y[:100], c.predict(x[:100]))
+    d.mlp.close()
+    m.close()
