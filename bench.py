@@ -320,6 +320,7 @@ def closure_leg(cfg, st0, args, ref, g_fine) -> dict:
 V100_TABLE_MS = {(64, 8): 0.76, (128, 16): 2.88, (256, 32): 24.2, (512, 32): 98.5,
                  (512, 64): 214.8, (1024, 64): 967.2}
 H200_256_MS = 150.0
+H200_512_MS = 41.9    # 512^2 x 64 open-top production configuration, 1 GPU
 
 
 def _time_steps(g, st, warmup: int, steps: int) -> tuple:
@@ -347,11 +348,12 @@ def _time_steps(g, st, warmup: int, steps: int) -> tuple:
 def published_configs_leg(args) -> dict:
     """The configurations behind the reference's published step times, run
     here as plain FP64 steps: the 3/2-rule 256^2 x 256 production branch
-    (H200: 150 ms/step) and the benchmark harness table (V100)."""
+    (H200: 150 ms/step), the open-top 512^2 x 64 production configuration
+    (H200: 41.9 ms/step) and the benchmark harness table (V100)."""
     import numpy as np
 
     from nhqg_amd import NHQGConfig, make_grid
-    from nhqg_amd.init import polar_config_256, polar_state_256, thermal_noise_state
+    from nhqg_amd.init import K_ONSET, polar_config_256, polar_state_256, thermal_noise_state
 
     out = {}
     cfg = polar_config_256(horizontal_dealiasing="32_rule", dt=5e-5)
@@ -365,6 +367,22 @@ def published_configs_leg(args) -> dict:
         "speedup_vs_published": H200_256_MS / syn,
         "finite": bool(np.isfinite(d["KE_tot"])),
         "source": "reference efficiency_review.md:3-6"}
+    g.close()
+    # the open-top production configuration of the reference's 512^2 campaign
+    L48 = 48.0 * 2.0 * np.pi / K_ONSET
+    cfg = polar_config_256(Nx=512, Nz=64, L=L48, gamma=0.0, sponge_rate=0.0,
+                           w_bc_top="neumann", dt=5e-5)
+    g = make_grid(cfg, shells="band")
+    thr, syn, d = _time_steps(g, thermal_noise_state(g, seed=0, amplitude=1e-3), 5, 50)
+    out["h200_opentop_512x512x64"] = {
+        "config": "balanced_sbp2_pc sub4, flux, evolve_mean, 2/3 rule, Neumann top (per-shell "
+                  "FP64 matrix solves), no trap, Nx=Ny=512, Nz=64, L=48 onset wavelengths, "
+                  "Ra=100, dt=5e-5",
+        "ms_per_step": thr, "ms_per_step_synced": syn,
+        "published_ms_per_step": H200_512_MS, "published_hardware": "1x H200 (JAX FP64)",
+        "speedup_vs_published": H200_512_MS / syn,
+        "finite": bool(np.isfinite(d["KE_tot"])),
+        "source": "reference polar_512_todo.md:217-226"}
     g.close()
     rows = []
     for (nx, nz), pub in V100_TABLE_MS.items():
