@@ -10,9 +10,9 @@
 // writing results.
 //
 // Per wave: 16 columns of B and the full column height of C -- MT row
-// tiles of 16 on v_mfma_f64_16x16x4_f64 plus, when M = 16*MT + 1, the last
-// row as a VALU dot product -- so each B value is read from HBM once and
-// each A fragment (from LDS) feeds one MFMA.  A is stored k-major and zero
+// tiles of 16 on v_mfma_f64_16x16x4_f64 plus, for 257-row operators
+// (MT = 16, M = 257), the last row as a VALU dot product -- so each B
+// value is read from HBM once and each A fragment (from LDS) feeds one MFMA.  A is stored k-major and zero
 // padded (GemmOp), so chunk loads are coalesced, the LDS stash is
 // conflict-free and no load needs a bounds branch: B rows beyond K are
 // clamped to row K-1 and meet zero columns of A.  B is prefetched R chunks
@@ -25,7 +25,7 @@
 #define GK_KC 8   // k-chunk staged in LDS per iteration (double buffered)
 #endif
 #ifndef GK_NW
-#define GK_NW 8   // waves per workgroup; 16 columns each
+#define GK_NW 4   // waves per workgroup; 16 columns each
 #endif
 #ifndef GK_R
 #define GK_R 2    // B ring depth in chunks
@@ -33,18 +33,18 @@
 
 static_assert(GK_KC % 4 == 0, "k-chunk must be a multiple of the MFMA depth");
 
-// Epilogue of tiles [t0, t0 + 4) of one lane (16 columns per wave, rows
+// Epilogue of tiles [t0, t0 + G) of one lane (16 columns per wave, rows
 // t*16 + r*4 + lk): the same per-element arithmetic as epi_apply, but each
-// extra field is read for all 16 elements at once, so a term costs one
+// extra field is read for all 4G elements at once, so a term costs one
 // memory latency rather than one per element.  Rows beyond M are clamped
 // for the loads and not stored.
-template <int MT>
-NHQG_DEV void epi_store4(const Epi& e, double* C, const v4d (&acc)[MT], int t0, int M, int ld,
-                         int col, int lk, int mode) {
-  double o[4][4];
+template <int MT, int G>
+NHQG_DEV void epi_store(const Epi& e, double* C, const v4d (&acc)[MT], int t0, int M, int ld,
+                        int col, int lk, int mode) {
+  double o[G][4];
   const double sa = e.s_acc ? e.s_acc[mode] : 1.0;
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+  for (int t = 0; t < G; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       o[t][r] = e.a_acc * (t0 + t < MT ? acc[t0 + t < MT ? t0 + t : 0][r] : 0.0);
@@ -55,16 +55,16 @@ NHQG_DEV void epi_store4(const Epi& e, double* C, const v4d (&acc)[MT], int t0, 
     const double ax = e.aX[x];
     const bool hs = e.sX[x] != nullptr;
     const double sx = hs ? e.sX[x][mode] : 1.0;
-    double v[4][4];
+    double v[G][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < G; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = min((t0 + t) * 16 + r * 4 + lk, M - 1);
         v[t][r] = X[(size_t)row * ld + col];
       }
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < G; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         double w = ax * v[t][r];
@@ -74,7 +74,7 @@ NHQG_DEV void epi_store4(const Epi& e, double* C, const v4d (&acc)[MT], int t0, 
   }
   const double so = e.s_out ? e.s_out[mode] : 1.0;
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+  for (int t = 0; t < G; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = (t0 + t) * 16 + r * 4 + lk;
@@ -87,8 +87,20 @@ NHQG_DEV void epi_store4(const Epi& e, double* C, const v4d (&acc)[MT], int t0, 
     }
 }
 
+// Operators of at most 96 rows (Nz <= 95) have short k loops, so a launch is
+// dominated by load latency and the epilogue: their kernels keep one row
+// tile per epilogue group and ask for 4-5 waves per SIMD (no spills at
+// these bounds), where the 4-tile epilogue needs ~160 VGPRs (3 waves).  On
+// the open-top 512^2 x 64 step the class went from 2.28 to 1.90 ms with
+// this and 4-wave workgroups (3 -> 5 resident per CU).
+template <int MT, bool UPD>
+constexpr int gk_min_waves() { return MT <= 5 ? (UPD ? 4 : 5) : MT == 6 ? 4 : 1; }
+template <int MT>
+constexpr int gk_epi_group() { return MT <= 6 ? 1 : 4; }
+
 template <int MT, bool UPD>
 __global__ void __launch_bounds__(GK_NW * 64)
+__attribute__((amdgpu_waves_per_eu(gk_min_waves<MT, UPD>(), 8)))
 gemm_f64_kernel(GemmBatch bt) {
   constexpr int NT = GK_NW * 64;
   constexpr int KC = GK_KC;
@@ -98,6 +110,9 @@ gemm_f64_kernel(GemmBatch bt) {
   constexpr int NA = (MP * KC + NT - 1) / NT;
   constexpr bool FULL = (NA * NT == MP * KC);
   constexpr int KS = KC / 4;
+  // the VALU last row exists only for 257-row operators (MT = 16); below
+  // that the launcher rejects M > 16 * MT
+  constexpr bool LAST = MT == 16;
   __shared__ double As[2][KC][SA];
 
   const GemmJob& J = bt.job[blockIdx.y];
@@ -117,7 +132,7 @@ gemm_f64_kernel(GemmBatch bt) {
   for (int t = 0; t < MT; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
   double last = 0.0;
   double ra[NA];
-  double bq[R][KS], lq[R][KS];
+  double bq[R][KS], lq[LAST ? R : 1][KS];
   double wq[UPD ? R : 1][KS], gq[UPD ? R : 1][KS];
 
   auto fetchA = [&](int ch) {
@@ -142,7 +157,7 @@ gemm_f64_kernel(GemmBatch bt) {
       const int kk = ch * KC + 4 * s + lk;
       const int kb = min(kk, K - 1);
       bq[slot][s] = B[(size_t)kb * ld + col];
-      lq[slot][s] = At[(size_t)kk * lda + MP];
+      if constexpr (LAST) lq[slot][s] = At[(size_t)kk * lda + MP];
       if constexpr (UPD) {
         wq[slot][s] = J.W[(size_t)kb * ld + col];
         gq[slot][s] = J.g[kb];
@@ -178,7 +193,7 @@ gemm_f64_kernel(GemmBatch bt) {
             const double a = As[cur][kr][t * 16 + li];
             acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
           }
-          last = fma(lq[j][s], bv, last);
+          if constexpr (LAST) last = fma(lq[j][s], bv, last);
         }
         stashA(cur ^ 1);
         __syncthreads();
@@ -189,13 +204,14 @@ gemm_f64_kernel(GemmBatch bt) {
   // v_mfma_f64_16x16x4_f64 output: D[i][j] of a tile sits in lane
   // j + 16*(i%4), register i/4 (one row group per register).
   const int mode = col >> 1;
+  constexpr int G = gk_epi_group<MT>();
   if (J.C2) {
 #pragma unroll
-    for (int t0 = 0; t0 < MT; t0 += 4) epi_store4<MT>(J.e2, J.C2, acc, t0, M, ld, col, lk, mode);
+    for (int t0 = 0; t0 < MT; t0 += G) epi_store<MT, G>(J.e2, J.C2, acc, t0, M, ld, col, lk, mode);
   }
 #pragma unroll
-  for (int t0 = 0; t0 < MT; t0 += 4) epi_store4<MT>(J.e, J.C, acc, t0, M, ld, col, lk, mode);
-  if (M > MP) {
+  for (int t0 = 0; t0 < MT; t0 += G) epi_store<MT, G>(J.e, J.C, acc, t0, M, ld, col, lk, mode);
+  if (LAST && M > MP) {
     // the four k-residue partial sums sit in lanes li, li+16, li+32, li+48
     last += __shfl_xor(last, 16);
     last += __shfl_xor(last, 32);
@@ -236,7 +252,7 @@ hipError_t launch_gemm_batch(const GemmBatch& b, int nj, hipStream_t s) {
   bool upd = false;
   for (int j = 0; j < nj; ++j) {
     const GemmOp& op = b.job[j].op;
-    if (op.mt != mt || op.M < 1 || op.M > mt * 16 + 1 || op.K < 1 || op.kp % GK_KC != 0 ||
+    if (op.mt != mt || op.M < 1 || op.M > mt * 16 + (mt == 16) || op.K < 1 || op.kp % GK_KC != 0 ||
         op.kp < op.K || op.lda != gemm_op_lda(mt) || !op.At || !b.job[j].B || !b.job[j].C)
       return hipErrorInvalidValue;
     const bool u = b.job[j].W != nullptr;
