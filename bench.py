@@ -99,9 +99,15 @@ def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
     t = sum(prof[k][0] for k in ("flux_col_inverse", "flux_row_products", "flux_col_forward"))
     nrhs = prof["flux_col_inverse"][1]
     # q, w, theta nodal in + 3 tendencies out; with the sponge also the
-    # coefficient levels of q, w, theta in and their 3 sponge terms out
+    # coefficient levels of q, w, theta in and their 3 sponge terms out, or
+    # of q alone where the w and theta sponges act on the nodal rows inside
+    # the pipeline (flux form, 2/3 rule, Nx = 256)
     sponge = g.sponge_phys is not None or g.sponge_phys_pad is not None
-    alg = 16.0 * nm * nrhs * (6.0 * nq + (2.0 * (nq + 2 * ng) if sponge else 0.0))
+    nodal = (sponge and g.Nx == 256 and g.nonlinear_advection == "flux"
+             and g.horizontal_dealiasing == "23_rule"
+             and os.environ.get("NHQG_NODAL_SPONGE", "1") != "0")
+    sp_levels = (nq if nodal else nq + 2 * ng) if sponge else 0
+    alg = 16.0 * nm * nrhs * (6.0 * nq + 2.0 * sp_levels)
     ach = alg / (t * 1e-3) / 1e9
     out["flux"] = dict(
         kernel="col_inverse_flux + row_flux2 + col_forward_flux (one FFT-product pipeline per RHS)",
@@ -109,7 +115,10 @@ def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
         achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s", frac=ach / PEAK_HBM_GBS,
         work_per_launch=alg / (3 * nrhs),
         work="per RHS: q, w, theta nodal in and their 3 tendencies out, (Nz+1) levels each"
-             + ("; with the sponge also the coefficient levels of q (Nz+1), w and theta (Nz-1 "
+             + ("; with the sponge also the coefficient levels of q (Nz+1) in and its sponge "
+                "term out (the w and theta sponges act on their nodal rows inside the pipeline)"
+                if nodal else
+                "; with the sponge also the coefficient levels of q (Nz+1), w and theta (Nz-1 "
                 "each) in and their 3 sponge terms out" if sponge else "")
              + "; x band modes x 16 B",
         traffic=profiled_traffic("flux") if full_size else None)

@@ -8,6 +8,7 @@
 #include <limits>
 #include <vector>
 #include <cstring>
+#include <cstdlib>
 #include <cstdio>
 #include <cmath>
 #include <algorithm>
@@ -114,6 +115,9 @@ struct nhqg_ctx {
   // while the step's contractions are enqueued (diagnostics stay FP64)
   int mixed_slices = 0;
   bool mixed_now = false;
+  // w and theta sponges on the nodal rows of the flux products where the
+  // row pass supports it (row_flux16); false: all three on coefficient levels
+  bool nodal_sponge = true;
 };
 
 namespace {
@@ -446,9 +450,12 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   a.X5 = c->X5;
   a.Y6 = c->Y6;
   a.fstride = c->fstride;
+  // flux form at N = 256: the sponge of w and theta on their nodal rows
+  // inside the row pass (fft_r16.hip), only q' on its coefficient levels
+  a.nsp = (c->nodal_sponge && a.sponge_phys && row_flux16_fits(a)) ? 1 : 0;
   const double F = 16.0 * c->Nq * (double)c->Npad * c->nky;   // one intermediate field
   const double Fc = 16.0 * c->Nq * (double)c->nm;            // one compact field
-  const int sp = a.sponge_phys ? 3 : 0;
+  const int sp = a.sponge_phys ? (a.nsp ? 1 : 3) : 0;
   // intermediates per pass: flux form 5 in / 6 out, Jacobian form 8 in / 3 out
   const int nxin = a.jac ? 8 : 5, nyout = a.jac ? 3 : 6;
   PL(CL_FLUX_COLINV, (3 + sp) * Fc + (nxin + sp) * F, launch_flux(a, 1, c->st));
@@ -464,8 +471,10 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   Epi eq = epi_make(-1.0), ew = epi_make(-1.0), et = epi_make(-1.0);
   if (a.sponge_phys) {
     epi_add(eq, -1.0, c->spq);
-    epi_add(ew, -1.0, c->spw);
-    epi_add(et, -1.0, c->spth);
+    if (!a.nsp) {
+      epi_add(ew, -1.0, c->spw);
+      epi_add(et, -1.0, c->spth);
+    }
   }
   if (th_tail) {
     // Eth receives th_scale * E_th + (the tail's terms) instead of E_th
@@ -490,7 +499,7 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
       if (f == 0 && !q_final) continue;
       Epi e2 = epi_make(-al);
       epi_add(e2, 1.0, ys[f]);
-      if (a.sponge_phys) epi_add(e2, -al, sps[f]);
+      if (a.sponge_phys && (f == 0 || !a.nsp)) epi_add(e2, -al, sps[f]);
       js[f]->C2 = R[f];
       js[f]->e2 = e2;
       if (fused) *fused |= 1 << f;
@@ -1148,6 +1157,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   // int8 slices of the Chebyshev transforms and the implicit-solve maps
   // (the SBP transfers, Psbp and the diagnostics operators stay FP64)
   c->mixed_slices = p->mixed_slices;
+  if (const char* e = std::getenv("NHQG_NODAL_SPONGE")) c->nodal_sponge = std::atoi(e) != 0;
   {
     const int S = c->mixed_slices;
     RUN(upload_op_i8(c, &o.V, h->V, S));

@@ -157,6 +157,9 @@ struct FluxArgs {
   // the trap's rows in X5's form (launch_eta_rows), or null: the Nx = 256
   // flux-form row pass adds eta to q' in spectral form when it is given
   const double2* eta_rows = nullptr;
+  // nodal sponge of w and theta (flux form at N = 256 with a sponge, see
+  // row_flux16): no sponge outputs sw / sth; the sponge is part of aw / ath
+  int nsp = 0;
 };
 // pass: 0 all three passes, 1 col_inverse, 2 row products, 3 col_forward
 hipError_t launch_flux(const FluxArgs& a, int pass, hipStream_t s);
@@ -177,6 +180,8 @@ hipError_t launch_init_put_q0(int nm, const int* full_idx, const double2* spec, 
 // the flux-form row pass at N = 256 with 16 threads per transform
 // (fft_r16.hip); false when a does not fit it
 bool launch_row_flux16(const FluxArgs& a, hipStream_t s);
+// whether a fits the 16-thread row pass (flux form, N = 256, 86 kept ky)
+bool row_flux16_fits(const FluxArgs& a);
 // out[x][ky] = N * sum_y eta_phys[x][y] exp(-2 pi i ky y / N), ky < nky
 hipError_t launch_eta_rows(int N, int nky, const double* eta_phys, const double2* twg,
                            double2* out, hipStream_t s);

@@ -30,6 +30,10 @@
 // acting on nodal values instead is equal in exact arithmetic, but the
 // transform back to coefficients then leaves roundoff of the large nodal
 // products in the tiny high Chebyshev coefficients, which d/dZ amplifies.
+// That matters for q' (its nodal values carry the barotropic part: ~6e-12
+// of w per step), not for w and theta (~1e-15): at N = 256 in flux form
+// (NS) their sponges act on the nodal rows inside the row pass (fft_r16.hip)
+// and only q' keeps a coefficient slot (X5 slot 5, Y6 slot 6).
 #include "fft_team.h"
 
 #ifndef NHQG_ROW16
@@ -54,7 +58,7 @@ struct ColBlock {
 // (Q, iKx Q, W, iKx W, Theta, iKx Theta; i ky is applied along y in pass 2),
 // then the sponge inputs.
 // ---------------------------------------------------------------------------
-template <int N, bool SPONGE, bool JAC>
+template <int N, bool SPONGE, bool JAC, bool NS = false>
 __global__ void __launch_bounds__(ColBlock<N>::v)
 col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ qn,
                  const double2* __restrict__ wn, const double2* __restrict__ thn,
@@ -66,7 +70,8 @@ col_inverse_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
   constexpr int BS = ColBlock<N>::v;
   constexpr int T = N / 4, TEAMS = BS / T;
   constexpr int NA = JAC ? 8 : 5;   // advection inputs
-  constexpr int NF = NA + (SPONGE ? 3 : 0);
+  // with NS only q' has a coefficient-level sponge slot
+  constexpr int NF = NA + (SPONGE ? (NS ? 1 : 3) : 0);
   __shared__ double2 buf[2][TEAMS][N + FFT_PAD];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
@@ -520,7 +525,7 @@ row_jac(int nky, int ng, const double2* __restrict__ X, size_t fstride, double2*
 // kept rows (Jacobian form: A_f = F_x(G_f)), times the output mask.  Writes
 // A into the nodal buffers.
 // ---------------------------------------------------------------------------
-template <int N, bool SPONGE, bool JAC>
+template <int N, bool SPONGE, bool JAC, bool NS = false>
 __global__ void __launch_bounds__(ColBlock<N>::v)
 col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ Y6,
                  size_t fstride, const int* __restrict__ rows, const double* __restrict__ kx_row,
@@ -556,8 +561,9 @@ col_forward_flux(int nkx, int nky, int ldc, int ng, const double2* __restrict__ 
     kxv[j] = in ? kx_row[i] : 0.0;
     mkv[j] = (in && active) ? mask[ky * nkx + i] : 0.0;
   }
-  // the sponge slot of w and theta exists only on their ng levels
-  auto valid = [&](int q) { return !(q % NG == NA && q / NG > 0 && l >= ng); };
+  // the sponge slot of w and theta exists only on their ng levels (and not
+  // at all with NS, where their sponge is part of the H slot)
+  auto valid = [&](int q) { return !(q % NG == NA && q / NG > 0 && (NS || l >= ng)); };
   auto gather = [&](int q, double2 (&v)[NL]) {
     const int f = q / NG, h = q % NG;
     const int slot = JAC ? (h == 0 ? f : 3 + f) : (h < 2) ? 2 * f + h : 6 + f;
@@ -698,7 +704,7 @@ row_wth(int nky, const double2* __restrict__ X5, size_t fstride, double* __restr
 
 
 
-template <int N, bool SP, bool JAC>
+template <int N, bool SP, bool JAC, bool NS = false>
 static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
   constexpr int TEAMS = 1024 / N;                       // row passes
   constexpr int BC = ColBlock<N>::v, TC = 4 * BC / N;   // column passes
@@ -706,9 +712,9 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
   const dim3 gc = xcd_grid(nb);
   dim3 gr((N + TEAMS - 1) / TEAMS, a.nq);
   if (pass == 0 || pass == 1)
-    hipLaunchKernelGGL((col_inverse_flux<N, SP, JAC>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc,
-                       a.ng, a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row, a.ky_col,
-                       a.inv_denom, a.tw, a.X5, a.fstride, nb);
+    hipLaunchKernelGGL((col_inverse_flux<N, SP, JAC, NS>), gc, dim3(BC), 0, s, a.nkx, a.nky,
+                       a.ldc, a.ng, a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row,
+                       a.ky_col, a.inv_denom, a.tw, a.X5, a.fstride, nb);
   if ((pass == 0 || pass == 2) && JAC) {
     if (a.eta_x)
       hipLaunchKernelGGL((row_jac<N, SP, true>), gr, dim3(256), 0, s, a.nky, a.ng, a.X5,
@@ -726,13 +732,22 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
                          a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
   }
   if (pass == 0 || pass == 3)
-    hipLaunchKernelGGL((col_forward_flux<N, SP, JAC>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc,
-                       a.ng, a.Y6, a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath, a.sq,
-                       a.sw, a.sth, a.tw, nb);
+    hipLaunchKernelGGL((col_forward_flux<N, SP, JAC, NS>), gc, dim3(BC), 0, s, a.nkx, a.nky,
+                       a.ldc, a.ng, a.Y6, a.fstride, a.rows, a.kx_row, a.mask, a.aq, a.aw, a.ath,
+                       a.sq, a.sw, a.sth, a.tw, nb);
 }
 
 template <int N>
 static hipError_t flux_n(const FluxArgs& a, int pass, hipStream_t s) {
+  if (a.nsp) {
+    // the nodal sponge rides in the 16-thread row pass only
+    if constexpr (N == 256) {
+      if (!NHQG_ROW16 || !a.sponge_phys || !row_flux16_fits(a)) return hipErrorInvalidValue;
+      flux_pass<N, true, false, true>(a, pass, s);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
   if (a.jac) {
     if (a.sponge_phys) flux_pass<N, true, true>(a, pass, s);
     else flux_pass<N, false, true>(a, pass, s);

@@ -122,7 +122,21 @@ NHQG_DEV void fft256(double2 (&v)[16], double2* xs, int t, const double2* tws) {
 // packing forms from the same values, belong to lane 16 - t and go there by
 // shuffles.  That halves the loads and leaves the registers for the
 // prefetch at two waves per SIMD.
-template <int nky, bool SPONGE, bool TRAP>
+//
+// NS (nodal sponge, with SPONGE): sigma acts on the nodal w and theta rows
+// that the products already hold, and F_y(sigma f) is added to the H_f slot
+// (H_f = i ky F_y(v f) + F_y(sigma f)); only q' keeps a coefficient-level
+// sponge slot (X5 slot 5 -> Y6 slot 6).  The sponge is linear and acts on
+// each level alike, so it commutes with the vertical maps: Pw S(VSw w) =
+// S(w) in exact arithmetic, and the roundoff the nodal route leaves in the
+// back-transformed w and theta tendencies stays ~1e-15 of the field per
+// step.  q' keeps the coefficient route: its nodal values carry the large
+// barotropic part, and their roundoff in the small high Chebyshev
+// coefficients, amplified by d/dZ in the implicit solve, reached ~6e-12 of
+// w per step.  Then v is transformed a third time (sweeps u | v with q',
+// w | v with theta) so that v is dead when a field's two products are
+// transformed: 19 transforms per row pair, 6 X5 and 7 Y6 slots.
+template <int nky, bool SPONGE, bool TRAP, bool NS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 row_flux16(int ng, const double2* __restrict__ X5, size_t fstride,
            double2* __restrict__ Y6, const double* __restrict__ ky_col,
@@ -262,6 +276,98 @@ row_flux16(int ng, const double2* __restrict__ X5, size_t fstride,
     for (int a = 0; a < 16; ++a) z[a] = make_double2(vel[a].x * z[a].x, vel[a].y * z[a].y);
     forward_store(z, 2 * fi + sweep, sweep == 1, 0.5 * sc * sc);
   };
+  if constexpr (NS) {
+    // forward DFT of one row's packed products v = sc (v f) + i (sigma f)
+    // (sc a power of two), unpacked into sc^2 F_y(v f) (A) and sc
+    // F_y(sigma f) (B) at the kept ky of this lane: H = i ky A + B -> dst
+    auto row_flux_sponge = [&](double2 (&v)[16], double2* dst) __attribute__((always_inline)) {
+      fft256<false>(v, xs, t, w);
+      const double half = 0.5 * sc;
+#pragma unroll
+      for (int c = 0; c < J; ++c) {
+        const double2 zo = make_double2(__shfl(v[15 - c].x, mirror, 64),
+                                        __shfl(v[15 - c].y, mirror, 64));
+        const double2 zs = v[(16 - c) & 15];
+        const double2 zm = make_double2(t == 0 ? zs.x : zo.x, t == 0 ? zs.y : zo.y);
+        const double2 zk = v[c];
+        const int k = t + 16 * c;
+        const double ky = kys[k];
+        const double2 A = make_double2(half * (zk.x + zm.x), half * (zk.y - zm.y));
+        const double2 B = make_double2(half * (zk.y + zm.y), half * (zm.x - zk.x));
+        if (k < nky) dst[16 * c] = make_double2(B.x - ky * A.y, B.y + ky * A.x);
+      }
+    };
+    // z: the field's physical rows x0 (re) and x0 + 1 (im), vel: v's; e:
+    // sigma on the two rows.  H_f of both rows -> Y6 slot `slot`
+    auto flux_sponge_store = [&](int slot, const double (&e)[32]) __attribute__((always_inline)) {
+      double2 c1[16];
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        const double2 f = z[a];
+        z[a] = make_double2(sc * (vel[a].x * f.x), e[a] * f.x);
+        c1[a] = make_double2(sc * (vel[a].y * f.y), e[16 + a] * f.y);
+      }
+      double2* d = Y6 + slot * fstride + lb + orow;
+      row_flux_sponge(z, d);
+      row_flux_sponge(c1, d + nky);
+    };
+    // sweep 0: u with q' + eta, w, theta (G slots), as without NS
+    build(z, false);
+    load_raw(2);
+    if (TRAP) load_trap();
+    inverse_t();
+#pragma unroll
+    for (int a = 0; a < 16; ++a) vel[a] = z[a];
+    build(z, TRAP);
+    load_raw(3);
+    inverse_t();
+    product_store(0, 0);
+#pragma unroll 1
+    for (int fi = 1; fi < 3; ++fi) {
+      build(z, false);
+      load_raw(fi == 1 ? 4 : 1);
+      inverse_t();
+      product_store(fi, 0);
+    }
+    // sweep 1: v with q' + eta (H slot), then w with its sponge (H slot)
+    build(z, false);
+    load_raw(2);
+    if (TRAP) load_trap();
+    inverse_t();
+#pragma unroll
+    for (int a = 0; a < 16; ++a) vel[a] = z[a];
+    build(z, TRAP);
+    load_raw(3);
+    inverse_t();
+    product_store(0, 1);
+    // sigma's rows and the next prefetch after the transform: held across
+    // it they would not fit beside the two products
+    double e[32];
+    build(z, false);
+    inverse_t();
+    load_phys(sponge_phys, e);
+    flux_sponge_store(3, e);
+    load_raw(1);
+    // sweep 2: v again, theta with its sponge (H slot)
+    build(z, false);
+    load_raw(4);
+    inverse_t();
+#pragma unroll
+    for (int a = 0; a < 16; ++a) vel[a] = z[a];
+    build(z, false);
+    inverse_t();
+    load_phys(sponge_phys, e);
+    flux_sponge_store(5, e);
+    load_raw(5);
+    // sigma times the coefficient levels of q'
+    build(z, false);
+    load_phys(sponge_phys, e);
+    fft256<true>(z, xs, t, w);
+#pragma unroll
+    for (int a = 0; a < 16; ++a) z[a] = make_double2(z[a].x * e[a], z[a].y * e[16 + a]);
+    forward_store(z, 6, false, 0.5 * sc);
+    return;
+  }
 #pragma unroll 1
   for (int sweep = 0; sweep < 2; ++sweep) {
     // u, then v
@@ -335,24 +441,35 @@ hipError_t launch_eta_rows(int N, int nky, const double* eta_phys, const double2
   return hipGetLastError();
 }
 
-bool launch_row_flux16(const FluxArgs& a, hipStream_t s) {
+bool row_flux16_fits(const FluxArgs& a) {
   // the band-compact layout's 86 kept ky rows (ky <= 256/3)
+  return a.N == 256 && !a.jac && a.nky == 86 && !(a.eta_phys && !a.eta_rows);
+}
+
+bool launch_row_flux16(const FluxArgs& a, hipStream_t s) {
   constexpr int NKY = 86;
-  if (a.N != 256 || a.jac || a.nky != NKY || (a.eta_phys && !a.eta_rows)) return false;
+  if (!row_flux16_fits(a) || (a.nsp && !a.sponge_phys)) return false;
   const dim3 g(256 / (2 * R16_TEAMS), a.nq);
-  if (a.sponge_phys) {
+  if (a.nsp) {
     if (a.eta_rows)
-      hipLaunchKernelGGL((row_flux16<NKY, true, true>), g, dim3(256), 0, s, a.ng, a.X5,
+      hipLaunchKernelGGL((row_flux16<NKY, true, true, true>), g, dim3(256), 0, s, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_rows, a.sponge_phys, a.tw);
     else
-      hipLaunchKernelGGL((row_flux16<NKY, true, false>), g, dim3(256), 0, s, a.ng, a.X5,
+      hipLaunchKernelGGL((row_flux16<NKY, true, false, true>), g, dim3(256), 0, s, a.ng, a.X5,
+                         a.fstride, a.Y6, a.ky_col, a.eta_rows, a.sponge_phys, a.tw);
+  } else if (a.sponge_phys) {
+    if (a.eta_rows)
+      hipLaunchKernelGGL((row_flux16<NKY, true, true, false>), g, dim3(256), 0, s, a.ng, a.X5,
+                         a.fstride, a.Y6, a.ky_col, a.eta_rows, a.sponge_phys, a.tw);
+    else
+      hipLaunchKernelGGL((row_flux16<NKY, true, false, false>), g, dim3(256), 0, s, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_rows, a.sponge_phys, a.tw);
   } else {
     if (a.eta_rows)
-      hipLaunchKernelGGL((row_flux16<NKY, false, true>), g, dim3(256), 0, s, a.ng, a.X5,
+      hipLaunchKernelGGL((row_flux16<NKY, false, true, false>), g, dim3(256), 0, s, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_rows, a.sponge_phys, a.tw);
     else
-      hipLaunchKernelGGL((row_flux16<NKY, false, false>), g, dim3(256), 0, s, a.ng, a.X5,
+      hipLaunchKernelGGL((row_flux16<NKY, false, false, false>), g, dim3(256), 0, s, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_rows, a.sponge_phys, a.tw);
   }
   return true;
