@@ -118,6 +118,8 @@ struct nhqg_ctx {
   // w and theta sponges on the nodal rows of the flux products where the
   // row pass supports it (row_flux16); false: all three on coefficient levels
   bool nodal_sponge = true;
+  // stage 2's V q1 and VSw w1 beside the stage-1 corrector preamble
+  bool qw_early = true;
 };
 
 namespace {
@@ -398,10 +400,15 @@ int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
 int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, double* Eq,
                  double* Ew, double* Eth, double* const* R = nullptr, double al = 0.0,
                  int* fused = nullptr, const Epi* th_tail = nullptr, double th_scale = 0.0,
-                 const double* thb = nullptr, double* Etb = nullptr) {
+                 const double* thb = nullptr, double* Etb = nullptr, bool qw_nodal = false) {
   const Ops& o = c->o;
-  RUN(gemm_jobs(c, {Job{&o.V, q, c->qn, epi_make(1.0)}, Job{&o.VSw, w, c->wn, epi_make(1.0)},
-                    Job{&o.VSd, th, c->thn, epi_make(1.0)}}));
+  // qw_nodal: qn = V q and wn = VSw w are already on the stream (the step
+  // issues them beside the stage-1 corrector's preamble)
+  if (qw_nodal)
+    RUN(gemm_jobs(c, {Job{&o.VSd, th, c->thn, epi_make(1.0)}}));
+  else
+    RUN(gemm_jobs(c, {Job{&o.V, q, c->qn, epi_make(1.0)}, Job{&o.VSw, w, c->wn, epi_make(1.0)},
+                      Job{&o.VSd, th, c->thn, epi_make(1.0)}}));
   const bool legacy = thb && Etb;
   if (legacy) {
     const int n = c->Nq;
@@ -758,6 +765,13 @@ int enqueue_step_body(nhqg_ctx* c) {
   } else {
     PL(CL_SMALL, 0.0, launch_small_matvec(Nq, Nq, o.proj_d, c->ythb, c->s1thb, c->st));
   }
+  // stage 2's nodal q1 and w1 do not depend on the corrector (it changes
+  // theta alone): with the preamble forked, the main stream computes them
+  // while the side stream runs the preamble's one-block kernels
+  const bool qw_early = em && fk && c->qw_early;
+  if (qw_early)
+    RUN(gemm_jobs(c, {Job{&o.V, c->s1q, c->qn, epi_make(1.0)},
+                      Job{&o.VSw, c->s1w, c->wn, epi_make(1.0)}}));
   // stage 1 corrector
   const double* th1 = c->s1th;
   const double* thb1 = c->s1thb;
@@ -782,7 +796,7 @@ int enqueue_step_body(nhqg_ctx* c) {
   }
   RUN(explicit_rhs(c, c->s1q, c->s1w, th1, c->E2q, c->E2w, rth_fused ? c->Rth : c->E2th, nullptr,
                    0.0, nullptr, rth_fused ? &htail : nullptr, dt * (1.0 - delta),
-                   legacy ? c->s1thb : nullptr, legacy ? c->etb2 : nullptr));
+                   legacy ? c->s1thb : nullptr, legacy ? c->etb2 : nullptr, qw_early));
   {
     // R_q2 = y + dt(delta E1 + (1-delta) E2) + om*(G_Z S_w w1) - om*diss_q*q1
     Epi e = epi_make(om);
@@ -1158,6 +1172,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   // (the SBP transfers, Psbp and the diagnostics operators stay FP64)
   c->mixed_slices = p->mixed_slices;
   if (const char* e = std::getenv("NHQG_NODAL_SPONGE")) c->nodal_sponge = std::atoi(e) != 0;
+  if (const char* e = std::getenv("NHQG_QW_EARLY")) c->qw_early = std::atoi(e) != 0;
   {
     const int S = c->mixed_slices;
     RUN(upload_op_i8(c, &o.V, h->V, S));
