@@ -314,7 +314,10 @@ NHQG_DEV void inv_pair_regs(double2* ba, double2* bb, int t, const double2* tw, 
 // The kernel is held at 3 waves per SIMD (amdgpu_waves_per_eu): 15% faster
 // than 2 waves (at N = 1024 it spills a few registers to stay there).
 // ---------------------------------------------------------------------------
-template <int N, bool SPONGE, bool TRAP>
+// With NS (nodal sponge, with SPONGE) sigma acts on the nodal w and theta
+// rows the products already hold and F_y(sigma f) is added into H_f; only
+// q' keeps its coefficient-level sponge slot (X5 slot 5 -> Y6 slot 6).
+template <int N, bool SPONGE, bool TRAP, bool NS = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, double2* __restrict__ Y6,
           const double* __restrict__ ky_col, const double* __restrict__ eta_phys,
@@ -397,13 +400,38 @@ row_flux2(int nky, int ng, const double2* __restrict__ X5, size_t fstride, doubl
       a1[j] = v1[j] * f1[j];
     }
     fwd_pair<N>(ba, bb, t, tw, a0, a1, nky, A, B);
-    store_pair(2 * f + 1, A, B, true);
+    if (NS && f > 0) {
+      // H_f = i ky F_y(v f) + F_y(sigma f) on the nodal rows
+      if (LATE) phys_rows(sponge_phys, s0, s1);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a0[j] = s0[j] * f0[j];
+        a1[j] = s1[j] * f1[j];
+      }
+      double2 A2[3], B2[3];
+      fwd_pair<N>(ba, bb, t, tw, a0, a1, nky, A2, B2);
+      if (active) {
+        double2* d0 = Y6 + (2 * f + 1) * fstride + r0;
+        double2* d1 = Y6 + (2 * f + 1) * fstride + r1;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int ky = t + q * T;
+          if (ky < nky) {
+            d0[ky] = cadd(cmul_i(A[q], ky_col[ky]), A2[q]);
+            d1[ky] = cadd(cmul_i(B[q], ky_col[ky]), B2[q]);
+          }
+        }
+      }
+    } else {
+      store_pair(2 * f + 1, A, B, true);
+    }
   }
   if (SPONGE) {
-    // sigma times each coefficient level of q', w, theta (spectral.py:53-60)
+    // sigma times each coefficient level of q', w, theta (spectral.py:53-60),
+    // of q' alone with NS
     if (LATE) phys_rows(sponge_phys, s0, s1);
 #pragma unroll 1
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < (NS ? 1 : 3); ++f) {
       if (f > 0 && l >= ng) break;  // block-uniform
       load_half<N>(hc, t, nky, X5 + (5 + f) * fstride + r0, X5 + (5 + f) * fstride + r1);
       double c0[4], c1[4], a0[4], a1[4];
@@ -725,11 +753,11 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
   } else if ((pass == 0 || pass == 2) && !(NHQG_ROW16 && N == 256 && launch_row_flux16(a, s))) {
     dim3 g2((N / 2 + TEAMS - 1) / TEAMS, a.nq);   // two rows per team
     if (a.eta_phys)
-      hipLaunchKernelGGL((row_flux2<N, SP, true>), g2, dim3(256), 0, s, a.nky, a.ng, a.X5,
+      hipLaunchKernelGGL((row_flux2<N, SP, true, NS>), g2, dim3(256), 0, s, a.nky, a.ng, a.X5,
                          a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
     else
-      hipLaunchKernelGGL((row_flux2<N, SP, false>), g2, dim3(256), 0, s, a.nky, a.ng, a.X5,
-                         a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+      hipLaunchKernelGGL((row_flux2<N, SP, false, NS>), g2, dim3(256), 0, s, a.nky, a.ng,
+                         a.X5, a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
   }
   if (pass == 0 || pass == 3)
     hipLaunchKernelGGL((col_forward_flux<N, SP, JAC, NS>), gc, dim3(BC), 0, s, a.nkx, a.nky,
@@ -740,13 +768,10 @@ static void flux_pass(const FluxArgs& a, int pass, hipStream_t s) {
 template <int N>
 static hipError_t flux_n(const FluxArgs& a, int pass, hipStream_t s) {
   if (a.nsp) {
-    // the nodal sponge rides in the 16-thread row pass only
-    if constexpr (N == 256) {
-      if (!NHQG_ROW16 || !a.sponge_phys || !row_flux16_fits(a)) return hipErrorInvalidValue;
-      flux_pass<N, true, false, true>(a, pass, s);
-      return hipGetLastError();
-    }
-    return hipErrorInvalidValue;
+    // flux form with a sponge only
+    if (a.jac || !a.sponge_phys) return hipErrorInvalidValue;
+    flux_pass<N, true, false, true>(a, pass, s);
+    return hipGetLastError();
   }
   if (a.jac) {
     if (a.sponge_phys) flux_pass<N, true, true>(a, pass, s);
