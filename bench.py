@@ -101,10 +101,9 @@ def rooflines(prof: dict, g, lay, full_size: bool, slices: int = 0) -> dict:
     # q, w, theta nodal in + 3 tendencies out; with the sponge also the
     # coefficient levels of q, w, theta in and their 3 sponge terms out, or
     # of q alone where the w and theta sponges act on the nodal rows inside
-    # the pipeline (flux form, 2/3 rule)
+    # the pipeline (flux form)
     sponge = g.sponge_phys is not None or g.sponge_phys_pad is not None
     nodal = (sponge and g.nonlinear_advection == "flux"
-             and g.horizontal_dealiasing == "23_rule"
              and os.environ.get("NHQG_NODAL_SPONGE", "1") != "0")
     sp_levels = (nq if nodal else nq + 2 * ng) if sponge else 0
     alg = 16.0 * nm * nrhs * (6.0 * nq + 2.0 * sp_levels)

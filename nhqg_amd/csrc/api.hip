@@ -457,9 +457,9 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
   a.X5 = c->X5;
   a.Y6 = c->Y6;
   a.fstride = c->fstride;
-  // flux form, 2/3 rule: the sponge of w and theta on their nodal rows
-  // inside the row pass, only q' on its coefficient levels
-  a.nsp = (c->nodal_sponge && a.sponge_phys && !a.jac && c->Npad == c->Nx) ? 1 : 0;
+  // flux form: the sponge of w and theta on their nodal rows inside the
+  // row pass, only q' on its coefficient levels
+  a.nsp = (c->nodal_sponge && a.sponge_phys && !a.jac) ? 1 : 0;
   const double F = 16.0 * c->Nq * (double)c->Npad * c->nky;   // one intermediate field
   const double Fc = 16.0 * c->Nq * (double)c->nm;            // one compact field
   const int sp = a.sponge_phys ? (a.nsp ? 1 : 3) : 0;

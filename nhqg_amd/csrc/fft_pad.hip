@@ -204,7 +204,7 @@ NHQG_DEV void fwd3_pair(double2* ba, double2* bb, int t, const double2* tw3, con
 // [ky] for the M padded rows x.  kidx_of_x (M entries): kept row of padded
 // row x, or -1 for the zero rows.
 // ---------------------------------------------------------------------------
-template <int P, bool SPONGE, bool JAC>
+template <int P, bool SPONGE, bool JAC, bool NS = false>
 __global__ void __launch_bounds__(ColBlock3<P>::v)
 col_inverse_pad(int nkx, int nky, int ldc, int ng, const double2* __restrict__ qn,
                 const double2* __restrict__ wn, const double2* __restrict__ thn,
@@ -216,7 +216,7 @@ col_inverse_pad(int nkx, int nky, int ldc, int ng, const double2* __restrict__ q
   constexpr int M = 3 * P, T = 3 * P / 4;
   constexpr int BS = ColBlock3<P>::v, TEAMS = ColBlock3<P>::C;
   constexpr int NA = JAC ? 8 : 5;
-  constexpr int NF = NA + (SPONGE ? 3 : 0);
+  constexpr int NF = NA + (SPONGE ? (NS ? 1 : 3) : 0);   // NS: q' alone
   __shared__ double2 buf[2][TEAMS][M + FFT_PAD];
   __shared__ double2 tw3[2 * P], twP[P];
   const int tid = threadIdx.x, team = tid / T, t = tid % T;
@@ -269,8 +269,10 @@ col_inverse_pad(int nkx, int nky, int ldc, int ng, const double2* __restrict__ q
 // Pass 2 (flux form) on the padded grid: Y6 slots as row_flux2 (G_f =
 // F_y(u f), H_f = i ky F_y(v f), sponge F_y(sigma f)), rows x, x+1 of one
 // level packed in each complex transform.  eta_phys, sponge_phys: M x M.
+// NS (with SPONGE): the w and theta sponges act on their nodal padded rows
+// and F_y(sigma f) is added into H_f, as in row_flux2; q' keeps its slot.
 // ---------------------------------------------------------------------------
-template <int P, bool SPONGE, bool TRAP>
+template <int P, bool SPONGE, bool TRAP, bool NS = false>
 __global__ void __launch_bounds__(RowBlock3<P>::v)
 row_flux_pad(int nky, int ng, const double2* __restrict__ X5, size_t fstride,
              double2* __restrict__ Y6, const double* __restrict__ ky_col,
@@ -344,14 +346,40 @@ row_flux_pad(int nky, int ng, const double2* __restrict__ X5, size_t fstride,
       a1[j] = v1[j] * f1[j];
     }
     fwd3_pair<P>(ba, bb, t, tw3, twP, a0, a1, nky, A, B);
-    store_pair(2 * f + 1, A, B, true);
+    if (NS && f > 0) {
+      // H_f = i ky F_y(v f) + F_y(sigma f) on the nodal padded rows
+      double s0[4], s1[4];
+      phys_rows(sponge_phys, s0, s1);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a0[j] = s0[j] * f0[j];
+        a1[j] = s1[j] * f1[j];
+      }
+      double2 A2[2], B2[2];
+      fwd3_pair<P>(ba, bb, t, tw3, twP, a0, a1, nky, A2, B2);
+      if (active) {
+        double2* d0 = Y6 + (2 * f + 1) * fstride + r0;
+        double2* d1 = Y6 + (2 * f + 1) * fstride + r1;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int ky = t + s * T;
+          if (ky < nky) {
+            d0[ky] = cadd(cmul_i(A[s], ky_col[ky]), A2[s]);
+            d1[ky] = cadd(cmul_i(B[s], ky_col[ky]), B2[s]);
+          }
+        }
+      }
+    } else {
+      store_pair(2 * f + 1, A, B, true);
+    }
   }
   if (SPONGE) {
-    // sigma_pad times each coefficient level of q', w, theta (spectral.py:62-74)
+    // sigma_pad times each coefficient level of q', w, theta (spectral.py:62-74),
+    // of q' alone with NS
     double s0[4], s1[4];
     phys_rows(sponge_phys, s0, s1);
 #pragma unroll 1
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < (NS ? 1 : 3); ++f) {
       if (f > 0 && l >= ng) break;  // block-uniform
       double c0[4], c1[4], a0[4], a1[4];
       inv3_pair<P>(ba, bb, t, tw3, twP, nky, X5 + (5 + f) * fstride + r0,
@@ -449,7 +477,7 @@ row_jac_pad(int nky, int ng, const double2* __restrict__ X, size_t fstride,
 // F_x(G_f) + F_x(H_f)) (M/Nx)^2 at the Nx kept rows (rows[i]: padded row
 // of kept row i); Jacobian form A_f = F_x(G_f) (M/Nx)^2.
 // ---------------------------------------------------------------------------
-template <int P, bool SPONGE, bool JAC>
+template <int P, bool SPONGE, bool JAC, bool NS = false>
 __global__ void __launch_bounds__(ColBlock3<P>::v)
 col_forward_pad(int nkx, int nky, int ldc, int ng, const double2* __restrict__ Y6,
                 size_t fstride, const int* __restrict__ rows, const double* __restrict__ kx_row,
@@ -482,7 +510,8 @@ col_forward_pad(int nkx, int nky, int ldc, int ng, const double2* __restrict__ Y
     rowv[j] = i < nkx ? rows[i] : 0;
     kxv[j] = i < nkx ? kx_row[i] : 0.0;
   }
-  auto valid = [&](int q) { return !(q % NG == NA && q / NG > 0 && l >= ng); };
+  // w and theta sponge slots: on their ng levels, and none with NS
+  auto valid = [&](int q) { return !(q % NG == NA && q / NG > 0 && (NS || l >= ng)); };
   double2 g[4];
 #pragma unroll 1
   for (int q = 0; q < NQ; ++q) {
@@ -527,7 +556,7 @@ col_forward_pad(int nkx, int nky, int ldc, int ng, const double2* __restrict__ Y
 // host launchers
 // ---------------------------------------------------------------------------
 
-template <int P, bool SP, bool JAC>
+template <int P, bool SP, bool JAC, bool NS = false>
 static void flux_pass_pad(const FluxArgs& a, int pass, hipStream_t s) {
   constexpr int M = 3 * P, T = 3 * P / 4;
   constexpr int BC = ColBlock3<P>::v, TC = ColBlock3<P>::C;
@@ -535,8 +564,8 @@ static void flux_pass_pad(const FluxArgs& a, int pass, hipStream_t s) {
   const int nb = (a.nky + TC - 1) / TC * a.nq;
   const dim3 gc = xcd_grid(nb);
   if (pass == 0 || pass == 1)
-    hipLaunchKernelGGL((col_inverse_pad<P, SP, JAC>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc,
-                       a.ng, a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row,
+    hipLaunchKernelGGL((col_inverse_pad<P, SP, JAC, NS>), gc, dim3(BC), 0, s, a.nkx, a.nky,
+                       a.ldc, a.ng, a.qn, a.wn, a.thn, a.qc, a.wc, a.thc, a.kidx_of_x, a.kx_row,
                        a.ky_col, a.inv_denom, a.tw, a.X5, a.fstride, nb);
   if ((pass == 0 || pass == 2) && JAC) {
     const dim3 gr((M + TR - 1) / TR, a.nq);
@@ -549,22 +578,28 @@ static void flux_pass_pad(const FluxArgs& a, int pass, hipStream_t s) {
   } else if (pass == 0 || pass == 2) {
     const dim3 g2((M / 2 + TR - 1) / TR, a.nq);   // two rows per team
     if (a.eta_phys)
-      hipLaunchKernelGGL((row_flux_pad<P, SP, true>), g2, dim3(BR), 0, s, a.nky, a.ng, a.X5,
-                         a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+      hipLaunchKernelGGL((row_flux_pad<P, SP, true, NS>), g2, dim3(BR), 0, s, a.nky, a.ng,
+                         a.X5, a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
     else
-      hipLaunchKernelGGL((row_flux_pad<P, SP, false>), g2, dim3(BR), 0, s, a.nky, a.ng, a.X5,
-                         a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
+      hipLaunchKernelGGL((row_flux_pad<P, SP, false, NS>), g2, dim3(BR), 0, s, a.nky, a.ng,
+                         a.X5, a.fstride, a.Y6, a.ky_col, a.eta_phys, a.sponge_phys, a.tw);
   }
   if (pass == 0 || pass == 3)
-    hipLaunchKernelGGL((col_forward_pad<P, SP, JAC>), gc, dim3(BC), 0, s, a.nkx, a.nky, a.ldc,
-                       a.ng, a.Y6, a.fstride, a.rows, a.kx_row, a.out_scale, a.aq, a.aw, a.ath,
-                       a.sq, a.sw, a.sth, a.tw, nb);
+    hipLaunchKernelGGL((col_forward_pad<P, SP, JAC, NS>), gc, dim3(BC), 0, s, a.nkx, a.nky,
+                       a.ldc, a.ng, a.Y6, a.fstride, a.rows, a.kx_row, a.out_scale, a.aq, a.aw,
+                       a.ath, a.sq, a.sw, a.sth, a.tw, nb);
 }
 
 template <int P>
 static hipError_t flux_pad_p(const FluxArgs& a, int pass, hipStream_t s) {
   // the layout must hold every kx row and ky <= Nx/2 (nkx = 2P, nky = P + 1)
   if (a.nkx != 2 * P || a.nky != P + 1) return hipErrorInvalidValue;
+  if (a.nsp) {
+    // flux form with a sponge only
+    if (a.jac || !a.sponge_phys) return hipErrorInvalidValue;
+    flux_pass_pad<P, true, false, true>(a, pass, s);
+    return hipGetLastError();
+  }
   if (a.jac) {
     if (a.sponge_phys) flux_pass_pad<P, true, true>(a, pass, s);
     else flux_pass_pad<P, false, true>(a, pass, s);

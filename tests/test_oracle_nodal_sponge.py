@@ -15,6 +15,7 @@ whose roundoff in the small high Chebyshev coefficients d/dZ amplifies.
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 from oracle import nhqg_oracle as O
 from tests.cases import active_state, trap_sponge_cfg
@@ -36,8 +37,12 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def test_nodal_sponge_of_w_and_theta_matches_the_coefficient_route():
-    cfg = trap_sponge_cfg()
+RULES = ("23_rule", "32_rule")
+
+
+@pytest.mark.parametrize("rule", RULES)
+def test_nodal_sponge_of_w_and_theta_matches_the_coefficient_route(rule):
+    cfg = trap_sponge_cfg(horizontal_dealiasing=rule)
     o = O.build_operators(cfg)
     s = active_state(cfg)
     nod = _nodal(o)
@@ -47,11 +52,12 @@ def test_nodal_sponge_of_w_and_theta_matches_the_coefficient_route():
         assert _rel(nod[name](f), ref) <= 1e-13, name
 
 
-def test_step_with_nodal_w_theta_sponge_tracks_the_reference_step():
+@pytest.mark.parametrize("rule", RULES)
+def test_step_with_nodal_w_theta_sponge_tracks_the_reference_step(rule):
     """One full imex_step with the w and theta sponges on the nodal route
     (q' on its coefficient levels, as on the device) within 1e-13 of the
-    reference's step, per field."""
-    cfg = trap_sponge_cfg()
+    reference's step, per field, under either dealiasing rule."""
+    cfg = trap_sponge_cfg(horizontal_dealiasing=rule)
     o = O.build_operators(cfg)
     s = active_state(cfg)
     ref = O.imex_step(s, o)
