@@ -52,6 +52,14 @@ struct Ops {
   // padded npad grid under the 3/2 rule): row map, kept rows, twiddles
   int *fkidx, *frows;
   double2* ftw;
+  // SBP substep transfer as a five-diagonal level stencil (when Psbp Z is
+  // banded): pb (Nq x 5), the corrector mean's diagonal weights per real
+  // column and its cross-term pairs on the self-conjugate columns
+  double* psbp_band = nullptr;
+  double* pwc = nullptr;
+  int2* xpairs = nullptr;
+  double* xpw = nullptr;
+  int nxpairs = 0;
 };
 
 }  // namespace
@@ -120,6 +128,10 @@ struct nhqg_ctx {
   bool nodal_sponge = true;
   // stage 2's V q1 and VSw w1 beside the stage-1 corrector preamble
   bool qw_early = true;
+  // SBP substeps by the banded stencil (o.psbp_band); Pband: its partial
+  // buffer in levels' worth of doubles (one mean + the block sums)
+  bool sbp_band = true;
+  int Pband = 0;
 };
 
 namespace {
@@ -277,11 +289,13 @@ enum {
   CL_ELEMENTWISE,
   CL_SMALL,
   CL_GEMM_I8,
+  CL_SBP_BAND,
   CL_COUNT
 };
 const char* const kClassNames[CL_COUNT] = {
     "gemm_f64_mfma", "flux_col_inverse", "flux_row_products", "flux_col_forward",
-    "shell_solve", "parseval_means", "sbp_serial", "elementwise", "small", "gemm_i8_mixed"};
+    "shell_solve", "parseval_means", "sbp_serial", "elementwise", "small", "gemm_i8_mixed",
+    "sbp_band"};
 
 void prof_begin(nhqg_ctx* c, hipEvent_t* a) {
   (void)hipEventCreate(a);
@@ -672,17 +686,31 @@ int corrector(nhqg_ctx* c, const double* w_in, const double* th_in, const double
   double* tb_b = thb_out;
   if (ns % 2 == 0) std::swap(tb_a, tb_b);
   PL(CL_SMALL, 0.0, launch_small_axpy(n, thb_in, thb_in, thb_in, 0.0, tb_a, c->st));
+  // with the banded transfer each substep but the last also sums the next
+  // substep's <w_sbp theta_sbp> (one value per level)
+  const bool band = o.psbp_band != nullptr;
+  int P = c->Pred;
   for (int s = 0; s < ns; ++s) {
-    PL(CL_PARSEVAL, 32.0 * n * (double)c->nm,
-       launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, cur, o.pw_corr, o.partner_corr, c->partial,
-                       c->st));
-    PL(CL_SBP, 0.0, launch_sbp_apply(n, c->Pred, c->partial, c->sbp_map, tb_a, tb_b, c->g, c->st));
+    if (s == 0 || !band) {
+      PL(CL_PARSEVAL, 32.0 * n * (double)c->nm,
+         launch_parseval(n, c->nm, c->ld, c->Pred, c->wsbp, cur, o.pw_corr, o.partner_corr,
+                         c->partial, c->st));
+      P = c->Pred;
+    }
+    PL(CL_SBP, 0.0, launch_sbp_apply(n, P, c->partial, c->sbp_map, tb_a, tb_b, c->g, c->st));
     std::swap(tb_a, tb_b);
-    Job j{s + 1 < ns ? &o.Psbp : &o.S2T, cur, s + 1 < ns ? nxt : th_out, epi_make(1.0)};
-    j.W = c->wsbp;
-    j.g = c->g;
-    j.bc = sdt;
-    RUN(gemm_jobs(c, {j}));
+    if (band && s + 1 < ns) {
+      PL(CL_SBP_BAND, 48.0 * n * (double)c->nm,
+         launch_sbp_band(n, c->ld, o.psbp_band, cur, c->wsbp, c->g, sdt, o.pwc, nxt, c->partial,
+                         c->partial + n, o.nxpairs, o.xpairs, o.xpw, c->st));
+      P = 1;
+    } else {
+      Job j{s + 1 < ns ? &o.Psbp : &o.S2T, cur, s + 1 < ns ? nxt : th_out, epi_make(1.0)};
+      j.W = c->wsbp;
+      j.g = c->g;
+      j.bc = sdt;
+      RUN(gemm_jobs(c, {j}));
+    }
     std::swap(cur, nxt);
   }
   // after the loop tb_a holds the last result
@@ -1128,9 +1156,11 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(upload_op_split(c, &o.W2S, h->W2S, false));
   RUN(upload_op_split(c, &o.T2S, h->T2S, false));
   RUN(upload_op_split(c, &o.S2T, h->S2T, true, true));
+  std::vector<double> Psbp_h;
   {
     // one SBP substep cycle: Psbp = T2S * S2T (Nq x Nq)
-    std::vector<double> P((size_t)Nq * Nq, 0.0);
+    std::vector<double>& P = Psbp_h;
+    P.assign((size_t)Nq * Nq, 0.0);
     for (int i = 0; i < Nq; ++i)
       for (int k = 0; k < Ng; ++k) {
         const double a = h->T2S[(size_t)i * Ng + k];
@@ -1173,6 +1203,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   c->mixed_slices = p->mixed_slices;
   if (const char* e = std::getenv("NHQG_NODAL_SPONGE")) c->nodal_sponge = std::atoi(e) != 0;
   if (const char* e = std::getenv("NHQG_QW_EARLY")) c->qw_early = std::atoi(e) != 0;
+  if (const char* e = std::getenv("NHQG_SBP_BAND")) c->sbp_band = std::atoi(e) != 0;
   {
     const int S = c->mixed_slices;
     RUN(upload_op_i8(c, &o.V, h->V, S));
@@ -1355,6 +1386,60 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   }
   o.pw_corr = pad ? o.pw_pad : o.pw;
   o.partner_corr = pad ? o.partner_pad : o.partner;
+  if (!Psbp_h.empty() && c->sbp_band) {
+    // Psbp Z (input rows 0 and Nq-1 zeroed by the substep update) as five
+    // diagonals when every other entry is construction roundoff
+    const std::vector<double>& P = Psbp_h;
+    double mx = 0.0;
+    for (double v : P) mx = std::max(mx, std::fabs(v));
+    bool band = mx > 0.0 && Nq >= 5;
+    for (int i = 0; i < Nq && band; ++i)
+      for (int j = 1; j < Nq - 1; ++j)
+        if (std::abs(i - j) > 2 && std::fabs(P[(size_t)i * Nq + j]) > 1e-12 * mx) {
+          band = false;
+          break;
+        }
+    if (band) {
+      std::vector<double> pb((size_t)Nq * 5, 0.0);
+      for (int i = 0; i < Nq; ++i)
+        for (int o2 = -2; o2 <= 2; ++o2) {
+          const int j = i + o2;
+          if (j >= 1 && j < Nq - 1) pb[(size_t)i * 5 + o2 + 2] = P[(size_t)i * Nq + j];
+        }
+      RUN(upload(c, &o.psbp_band, pb.data(), pb.size()));
+      // the corrector mean (parseval_partial_kernel with pw_corr and
+      // partner_corr) split into diagonal weights per real column and
+      // cross terms of the partnered (self-conjugate) columns
+      const double* hpw = (pad && h->pweight_pad) ? h->pweight_pad : h->pweight;
+      const int32_t* hpt = (pad && h->partner_pad) ? h->partner_pad : h->partner;
+      std::vector<double> pwc((size_t)c->ld, 0.0);
+      std::vector<int2> xp;
+      std::vector<double> xw;
+      for (int m = 0; m < nm; ++m) {
+        const int q = hpt ? hpt[m] : -1;
+        double d = hpw[m];
+        if (q == m) {
+          d = 0.5 * hpw[m];
+          xp.push_back(make_int2(m, m));
+          xw.push_back(0.25 * hpw[m]);
+        } else if (q >= 0) {
+          d = 0.25 * (hpw[m] + hpw[q]);
+          if (m < q) {
+            xp.push_back(make_int2(m, q));
+            xw.push_back(0.25 * (hpw[m] + hpw[q]));
+          }
+        }
+        pwc[2 * (size_t)m] = d;
+        pwc[2 * (size_t)m + 1] = d;
+      }
+      RUN(upload(c, &o.pwc, pwc.data(), pwc.size()));
+      o.nxpairs = (int)xp.size();
+      if (o.nxpairs) {
+        RUN(upload(c, &o.xpairs, xp.data(), xp.size()));
+        RUN(upload(c, &o.xpw, xw.data(), xw.size()));
+      }
+    }
+  }
   c->h_thb2s.assign(h->thb2s, h->thb2s + (size_t)Nq * Nq);
   c->h_d1b.assign(h->d1_band, h->d1_band + (size_t)Nq * 3);
 
@@ -1390,7 +1475,8 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   RUN(field(c, &c->spq, Nq));
   RUN(field(c, &c->spw, Ng));
   RUN(field(c, &c->spth, Ng));
-  RUN(dalloc(c, &c->partial, (size_t)Nq * c->Pred + 64));
+  c->Pband = (c->ld + 255) / 256 + 1;
+  RUN(dalloc(c, &c->partial, (size_t)Nq * std::max(c->Pred, c->Pband) + 64));
   RUN(dalloc(c, &c->sbp, (size_t)Nq * 16));
   RUN(dalloc(c, &c->sbp_xy, (size_t)Nq * 2 * Nq));
   RUN(dalloc(c, &c->sbp_map, (size_t)4 * Nq * Nq));

@@ -770,6 +770,117 @@ sbp_apply_kernel(int n, int P, const double* __restrict__ partial, const double*
   }
 }
 
+// ---------------------------------------------------------------------------
+// SBP substep transfer as a level stencil.  With the input's boundary rows
+// zeroed (Z), one substep cycle Psbp = T2S S2T (SBP grid -> Galerkin theta
+// -> SBP grid) is c2s s2c on the interior: the Dirichlet projection is the
+// identity on the CGL interpolant of values that vanish at both ends, and
+// the two linear interpolations compose to a band of half-width 2.  The
+// host keeps the five diagonals pb[i][o] (o = -2..2, input rows 1..n-2) when
+// every other entry of Psbp Z is roundoff (create_impl), and each substep
+// but the last is then
+//   out[i] = sum_o pb[i][o] x[i+o],  x[j] = Z_j (th[j] - bc g[j] w[j])
+// (the split kernel's fused update), with the next substep's horizontal
+// mean <w, out> summed on the way: bpart[i * nb + blockIdx.x] over the
+// block's columns with the diagonal Parseval weights pwc (per real column);
+// sbp_cross_kernel adds the nb block sums of a level and the
+// self-conjugate columns' cross terms into partial[i].  One thread per real column, LB levels per block (a window of
+// five x and three w in registers); a bandwidth-bound pass instead of a
+// 257 x 257 contraction.
+// ---------------------------------------------------------------------------
+#define SBP_BAND_LB 32
+__global__ void __launch_bounds__(256)
+sbp_band_kernel(int n, int ld, const double* __restrict__ pb, const double* __restrict__ th,
+                const double* __restrict__ w, const double* __restrict__ g, double bc,
+                const double* __restrict__ pwc, double* __restrict__ out,
+                double* __restrict__ bpart) {
+  __shared__ double red[4][SBP_BAND_LB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = blockIdx.x * 256 + tid;   // ld is a multiple of 64: every lane is a column
+  const bool on = c < ld;
+  const int cc = on ? c : ld - 1;
+  const int i0 = blockIdx.y * SBP_BAND_LB, i1 = min(n, i0 + SBP_BAND_LB);
+  const double wt = on ? pwc[cc] : 0.0;
+  auto ld_x = [&](int j, double& xv, double& wv) {
+    if (j < 0 || j >= n) {
+      xv = 0.0;
+      wv = 0.0;
+      return;
+    }
+    wv = w[(size_t)j * ld + cc];
+    const double t = th[(size_t)j * ld + cc];
+    xv = (j == 0 || j == n - 1) ? 0.0 : t - bc * g[j] * wv;
+  };
+  // window: x[i-2 .. i+2], w[i .. i+2]
+  double x0, x1, x2, x3, x4, w0, w1, w2, dummy;
+  ld_x(i0 - 2, x0, dummy);
+  ld_x(i0 - 1, x1, dummy);
+  ld_x(i0, x2, w0);
+  ld_x(i0 + 1, x3, w1);
+#pragma unroll 4
+  for (int i = i0; i < i1; ++i) {
+    ld_x(i + 2, x4, w2);
+    const double* p = pb + 5 * i;
+    const double v = p[0] * x0 + p[1] * x1 + p[2] * x2 + p[3] * x3 + p[4] * x4;
+    if (on) out[(size_t)i * ld + c] = v;
+    const double s = wave_sum(wt * w0 * v);
+    if (lane == 0) red[wave][i - i0] = s;
+    x0 = x1;
+    x1 = x2;
+    x2 = x3;
+    x3 = x4;
+    w0 = w1;
+    w1 = w2;
+  }
+  __syncthreads();
+  if (tid < i1 - i0)
+    bpart[(size_t)(i0 + tid) * gridDim.x + blockIdx.x] =
+        (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+}
+
+// Cross terms of the horizontal mean's Hermitian part on the self-conjugate
+// columns: for each pair (m, q) (q = m for a self-partnered mode) with
+// weight wt, wt (a_m.r b_q.r + a_q.r b_m.r - a_m.i b_q.i - a_q.i b_m.i);
+// together with the diagonal weights of sbp_band_kernel this is exactly
+// the sum of parseval_partial_kernel.  One block per level: the level's nb
+// block sums of sbp_band_kernel plus the cross terms -> partial[i].
+__global__ void __launch_bounds__(256)
+sbp_cross_kernel(int n, int ld, int npairs, const int2* __restrict__ pq,
+                 const double* __restrict__ pwt, const double* __restrict__ a,
+                 const double* __restrict__ b, const double* __restrict__ bpart, int nb,
+                 double* __restrict__ partial) {
+  __shared__ double red[4];
+  const int i = blockIdx.x, tid = threadIdx.x;
+  const double* ar = a + (size_t)i * ld;
+  const double* br = b + (size_t)i * ld;
+  double s = 0.0;
+  for (int k = tid; k < nb; k += 256) s += bpart[(size_t)i * nb + k];
+  for (int k = tid; k < npairs; k += 256) {
+    const int m = pq[k].x, q = pq[k].y;
+    const double amr = ar[2 * m], ami = ar[2 * m + 1], aqr = ar[2 * q], aqi = ar[2 * q + 1];
+    const double bmr = br[2 * m], bmi = br[2 * m + 1], bqr = br[2 * q], bqi = br[2 * q + 1];
+    s += pwt[k] * ((amr * bqr + aqr * bmr) - (ami * bqi + aqi * bmi));
+  }
+  s = wave_sum(s);
+  if ((tid & 63) == 0) red[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) partial[i] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+hipError_t launch_sbp_band(int n, int ld, const double* pb, const double* th, const double* w,
+                           const double* g, double bc, const double* pwc, double* out,
+                           double* partial, double* bpart, int npairs, const int2* pq,
+                           const double* pwt, hipStream_t s) {
+  const int ncb = (ld + 255) / 256;
+  if (ld % 64 != 0 || th == out) return hipErrorInvalidValue;
+  const dim3 grid(ncb, (n + SBP_BAND_LB - 1) / SBP_BAND_LB);
+  hipLaunchKernelGGL(sbp_band_kernel, grid, dim3(256), 0, s, n, ld, pb, th, w, g, bc, pwc, out,
+                     bpart);
+  hipLaunchKernelGGL(sbp_cross_kernel, dim3(n), dim3(256), 0, s, n, ld, npairs, pq, pwt, w, out,
+                     bpart, ncb, partial);
+  return hipGetLastError();
+}
+
 // sbp buffer (doubles, n = Nz+1): [0,5n) B band, [5n,10n) LU bands,
 // [10n,11n) reciprocal pivots.
 hipError_t launch_sbp_prepare(int n, int P, const double* partial, const double* d1b,

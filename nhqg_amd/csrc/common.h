@@ -219,6 +219,13 @@ hipError_t launch_sbp_maps(int n, const double* sbp, const double* thb2s, const 
 // one substep: th_dst, g from th_src and the Parseval partials
 hipError_t launch_sbp_apply(int n, int P, const double* partial, const double* Mall,
                             const double* th_src, double* th_dst, double* g_out, hipStream_t s);
+// one SBP substep transfer as a five-diagonal level stencil with the fused
+// update and the next substep's horizontal mean, one value per level in
+// partial (bpart: n * ceil(ld / 256) block sums of scratch; columns.hip)
+hipError_t launch_sbp_band(int n, int ld, const double* pb, const double* th, const double* w,
+                           const double* g, double bc, const double* pwc, double* out,
+                           double* partial, double* bpart, int npairs, const int2* pq,
+                           const double* pwt, hipStream_t s);
 hipError_t launch_small_matvec(int n_out, int n_in, const double* M, const double* x, double* y,
                                hipStream_t s);
 hipError_t launch_small_axpy(int n, const double* x0, const double* x1, const double* x2, double c,
