@@ -128,6 +128,8 @@ struct nhqg_ctx {
   bool nodal_sponge = true;
   // stage 2's V q1 and VSw w1 beside the stage-1 corrector preamble
   bool qw_early = true;
+  // stage 1 stores y + dt delta E1 in place of E1 where E1 is final
+  bool y1_tail = true;
   // SBP substeps by the banded stencil (o.psbp_band); Pband: its partial
   // buffer in levels' worth of doubles (one mean + the block sums)
   bool sbp_band = true;
@@ -414,7 +416,8 @@ int combine(nhqg_ctx* c, int nl, double* out, const Epi& e) {
 int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th, double* Eq,
                  double* Ew, double* Eth, double* const* R = nullptr, double al = 0.0,
                  int* fused = nullptr, const Epi* th_tail = nullptr, double th_scale = 0.0,
-                 const double* thb = nullptr, double* Etb = nullptr, bool qw_nodal = false) {
+                 const double* thb = nullptr, double* Etb = nullptr, bool qw_nodal = false,
+                 double ytail = 0.0, int* ytailed = nullptr) {
   const Ops& o = c->o;
   // qw_nodal: qn = V q and wn = VSw w are already on the stream (the step
   // issues them beside the stage-1 corrector's preamble)
@@ -507,6 +510,23 @@ int explicit_rhs(nhqg_ctx* c, const double* q, const double* w, const double* th
     for (int t = 0; t < et.nX; ++t) et.aX[t] *= th_scale;
     for (int t = 0; t < th_tail->nX; ++t)
       epi_add(et, th_tail->aX[t], th_tail->X[t], th_tail->sX[t]);
+  }
+  // ytail: store y + ytail * E instead of E for every field whose E is final
+  // here (q' not when the beta term or the closure add to it below); the
+  // fields done are flagged in *ytailed
+  if (ytailed) *ytailed = 0;
+  if (ytail != 0.0) {
+    Epi* es[3] = {&eq, &ew, &et};
+    const double* ys[3] = {q, w, th};
+    const bool q_ok = c->P.beta == 0.0 && !c->cl_on && !th_tail;
+    for (int f = 0; f < 3; ++f) {
+      if ((f == 0 && !q_ok) || (f == 2 && th_tail) || es[f]->nX + 1 > NHQG_EPI_TERMS) continue;
+      Epi& e = *es[f];
+      e.a_acc *= ytail;
+      for (int t = 0; t < e.nX; ++t) e.aX[t] *= ytail;
+      epi_add(e, 1.0, ys[f]);
+      if (ytailed) *ytailed |= 1 << f;
+    }
   }
   Job jq{&o.Vinv, c->qn, Eq, eq}, jw{&o.Pw, c->wn, Ew, ew}, jt{&o.Pd, c->thn, Eth, et};
   if (fused) *fused = 0;
@@ -766,8 +786,17 @@ int enqueue_step_body(nhqg_ctx* c) {
   // where E1 is final there
   double* const R1[3] = {c->Rq, c->Rw, c->Rth};
   int fused = 0;
+  // stage 2 uses E1 only as y + dt delta E1: where E1 is final in its back
+  // transform, that sum is stored in its place (E1 then holds Y1), which
+  // saves stage 2 the read of y
+  int yt = 0;
   RUN(explicit_rhs(c, c->yq, c->yw, c->yth, c->E1q, c->E1w, c->E1th, R1, al, &fused, nullptr,
-                   0.0, legacy ? c->ythb : nullptr, legacy ? c->etb1 : nullptr));
+                   0.0, legacy ? c->ythb : nullptr, legacy ? c->etb1 : nullptr, false,
+                   c->y1_tail ? dt * delta : 0.0, &yt));
+  if (yt & ~fused) {
+    c->err = "stage-1 tail on a field without its fused right-hand side";
+    return NHQG_ERR_ARG;
+  }
   {
     const double* ys[3] = {c->yq, c->yw, c->yth};
     const double* es[3] = {c->E1q, c->E1w, c->E1th};
@@ -813,8 +842,12 @@ int enqueue_step_body(nhqg_ctx* c) {
   const bool rth_fused = !c->P.w_neumann;
   Epi htail = epi_make(0.0);
   if (rth_fused) {
-    epi_add(htail, 1.0, c->yth);
-    epi_add(htail, dt * delta, c->E1th);
+    if (yt & 4) {
+      epi_add(htail, 1.0, c->E1th);
+    } else {
+      epi_add(htail, 1.0, c->yth);
+      epi_add(htail, dt * delta, c->E1th);
+    }
     epi_add(htail, om, c->s1w);
     epi_add(htail, -om, th1, o.diss_th);
     if (em) {
@@ -828,8 +861,12 @@ int enqueue_step_body(nhqg_ctx* c) {
   {
     // R_q2 = y + dt(delta E1 + (1-delta) E2) + om*(G_Z S_w w1) - om*diss_q*q1
     Epi e = epi_make(om);
-    epi_add(e, 1.0, c->yq);
-    epi_add(e, dt * delta, c->E1q);
+    if (yt & 1) {
+      epi_add(e, 1.0, c->E1q);
+    } else {
+      epi_add(e, 1.0, c->yq);
+      epi_add(e, dt * delta, c->E1q);
+    }
     epi_add(e, dt * (1.0 - delta), c->E2q);
     epi_add(e, -om, c->s1q, o.diss_q);
     // R_w2 = ... + om*(inv_denom Qw q1 + Ra_s Mtw th1) - om diss_w w1
@@ -840,8 +877,12 @@ int enqueue_step_body(nhqg_ctx* c) {
     }
     Epi f = epi_make(om);
     f.s_acc = o.inv_denom;
-    epi_add(f, 1.0, c->yw);
-    epi_add(f, dt * delta, c->E1w);
+    if (yt & 2) {
+      epi_add(f, 1.0, c->E1w);
+    } else {
+      epi_add(f, 1.0, c->yw);
+      epi_add(f, dt * delta, c->E1w);
+    }
     epi_add(f, dt * (1.0 - delta), c->E2w);
     epi_add(f, om * c->P.ra_sigma, thw);
     epi_add(f, -om, c->s1w, o.diss_w);
@@ -854,8 +895,12 @@ int enqueue_step_body(nhqg_ctx* c) {
         ith = c->tmpg;
       }
       Epi h = epi_make(0.0);
-      epi_add(h, 1.0, c->yth);
-      epi_add(h, dt * delta, c->E1th);
+      if (yt & 4) {
+        epi_add(h, 1.0, c->E1th);
+      } else {
+        epi_add(h, 1.0, c->yth);
+        epi_add(h, dt * delta, c->E1th);
+      }
       epi_add(h, dt * (1.0 - delta), c->E2th);
       epi_add(h, om, ith);
       epi_add(h, -om, th1, o.diss_th);
@@ -1203,6 +1248,7 @@ int create_impl(nhqg_ctx* c, const nhqg_params* p, const nhqg_operators* h) {
   c->mixed_slices = p->mixed_slices;
   if (const char* e = std::getenv("NHQG_NODAL_SPONGE")) c->nodal_sponge = std::atoi(e) != 0;
   if (const char* e = std::getenv("NHQG_QW_EARLY")) c->qw_early = std::atoi(e) != 0;
+  if (const char* e = std::getenv("NHQG_Y1_TAIL")) c->y1_tail = std::atoi(e) != 0;
   if (const char* e = std::getenv("NHQG_SBP_BAND")) c->sbp_band = std::atoi(e) != 0;
   {
     const int S = c->mixed_slices;
