@@ -755,9 +755,8 @@ int vertical_cutoff(nhqg_ctx* c, double* w, double* th) {
 
 int sanitize(nhqg_ctx* c) {
   const Ops& o = c->o;
-  PL(CL_SMALL, 0.0, launch_sanitize(c->Nq, c->ld, c->P.n_pairs, o.pairs, 0, c->yq, c->st));
-  PL(CL_SMALL, 0.0, launch_sanitize(c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yw, c->st));
-  PL(CL_SMALL, 0.0, launch_sanitize(c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yth, c->st));
+  PL(CL_SMALL, 0.0, launch_sanitize3(c->Nq, c->Ng, c->ld, c->P.n_pairs, o.pairs, 0, c->yq,
+                                     c->yw, c->yth, c->st));
   if (c->P.apply_mask) {
     LK(launch_mask(c->Nq, c->ld, c->yq, o.mask, c->st));
     LK(launch_mask(c->Ng, c->ld, c->yw, o.mask, c->st));

@@ -952,10 +952,22 @@ hipError_t launch_small_axpy(int n, const double* x0, const double* x1, const do
 // Hermitian-project the self-conjugate columns pairwise, and (full layout
 // only) multiply by the 2/3 mask.  pairs[p] = (m, m') with m <= m'.
 // ---------------------------------------------------------------------------
+// Rows of up to three fields in one launch: blockIdx.y < nl0 in f0, then nl1
+// rows of f1, then the rest of f2 (the same per-row arithmetic as three
+// launches).
 __global__ void __launch_bounds__(256)
-sanitize_kernel(int nl, int ld, int npairs, const int2* __restrict__ pairs, int m0,
-                double* __restrict__ f) {
-  const int l = blockIdx.y;
+sanitize_kernel(int nl0, int nl1, int ld, int npairs, const int2* __restrict__ pairs, int m0,
+                double* __restrict__ f0, double* __restrict__ f1, double* __restrict__ f2) {
+  int l = blockIdx.y;
+  double* f = f0;
+  if (l >= nl0) {
+    l -= nl0;
+    f = f1;
+    if (l >= nl1) {
+      l -= nl1;
+      f = f2;
+    }
+  }
   double* r = f + (size_t)l * ld;
   for (int p = blockIdx.x * 256 + threadIdx.x; p < npairs; p += gridDim.x * 256) {
     const int2 pr = pairs[p];
@@ -978,8 +990,16 @@ sanitize_kernel(int nl, int ld, int npairs, const int2* __restrict__ pairs, int 
 hipError_t launch_sanitize(int nl, int ld, int npairs, const int2* pairs, int m0, double* f,
                            hipStream_t s) {
   const int bx = (npairs + 255) / 256;
-  hipLaunchKernelGGL(sanitize_kernel, dim3(bx > 0 ? bx : 1, nl), dim3(256), 0, s, nl, ld, npairs,
-                     pairs, m0, f);
+  hipLaunchKernelGGL(sanitize_kernel, dim3(bx > 0 ? bx : 1, nl), dim3(256), 0, s, nl, 0, ld,
+                     npairs, pairs, m0, f, f, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_sanitize3(int nl0, int nl12, int ld, int npairs, const int2* pairs, int m0,
+                            double* f0, double* f1, double* f2, hipStream_t s) {
+  const int bx = (npairs + 255) / 256;
+  hipLaunchKernelGGL(sanitize_kernel, dim3(bx > 0 ? bx : 1, nl0 + 2 * nl12), dim3(256), 0, s,
+                     nl0, nl12, ld, npairs, pairs, m0, f0, f1, f2);
   return hipGetLastError();
 }
 

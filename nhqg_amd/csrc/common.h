@@ -232,6 +232,9 @@ hipError_t launch_small_axpy(int n, const double* x0, const double* x1, const do
                              double* y, hipStream_t s);
 hipError_t launch_sanitize(int nl, int ld, int npairs, const int2* pairs, int m0, double* f,
                            hipStream_t s);
+// the same on three fields in one launch: nl0 rows of f0, nl12 of f1 and f2
+hipError_t launch_sanitize3(int nl0, int nl12, int ld, int npairs, const int2* pairs, int m0,
+                            double* f0, double* f1, double* f2, hipStream_t s);
 hipError_t launch_mask(int nl, int ld, double* f, const double* mask, hipStream_t s);
 hipError_t launch_gather(int nl, int nm, int nm_pad, int full_n, const int* full_idx,
                          const double2* full, double2* comp, hipStream_t s);
